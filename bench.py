@@ -1,0 +1,225 @@
+"""Benchmark of the PIV window-correlation hot path on one MI355X.
+
+A step is one pass of the fused engine over one synthetic frame pair that is
+already resident in HBM: 4096 x 4096 float32 frames (particles d = 2.5 px,
+density 0.05 /px, I0 ~ U(100, 255), noise sigma 2, solid-body rotation with
+|d| = 6 px at the farthest window), 32 x 32 windows at step 16 (50%
+overlap): 65025 windows -> 65025 vectors per step.
+
+Timing: W untimed warm-up steps, then K launches bracketed by HIP events on
+the engine's own stream (opiv_ctx_stream) with device synchronisation on
+both sides.  Accuracy: a bounded sample of windows is compared with the FP64
+oracle and the max displacement error is reported.  The CPU baseline is the
+reference's own FP64 path (oracle/_ref/ref_piv, the reference's headers and
+thread pool) timed on this host's cores over the same frame pair, or the
+oracle restatement when the reference harness is absent.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--win 32]
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "openpiv-cxx_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "vectors/sec at 32² win 50% overlap, chosen vs plain GPU path at ≤1e-3 px"
+RESULT_BYTES = 72
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream through libamdhip64."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, e0, e1):
+        assert self.hip.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        return ms.value
+
+
+def workload(win: int):
+    import openpiv_amd as oa
+    w = 4096
+    step = win // 2
+    r = float(np.hypot(w / 2 - win / 2, w / 2 - win / 2))
+    a, b = oa.synth_pair(w, w, seed=2, field=oa.FIELD_ROTATION, params=(6.0, r), dtype=np.float32)
+    return a, b, w, step
+
+
+def cpu_baseline(a, b, win, step, threads):
+    """Reference FP64 path on host cores: vectors/s over the whole pair."""
+    ref_bin = ROOT / "oracle" / "_ref" / "ref_piv"
+    h, w = a.shape
+    nwin = (1 + (w - win) // step) * (1 + (h - win) // step)
+    if ref_bin.exists():
+        with tempfile.TemporaryDirectory() as d:
+            pa, pb = Path(d) / "a.bin", Path(d) / "b.bin"
+            a.astype(np.float64).tofile(pa)
+            b.astype(np.float64).tofile(pb)
+            reps = 4
+            out = subprocess.run([str(ref_bin), "time", str(pa), str(pb), str(w), str(h), str(win), str(step),
+                                  str(step), str(threads), str(reps)], capture_output=True, text=True,
+                                 check=True, timeout=600).stdout.split("\n")
+            us = sorted(float(l.split()[1]) for l in out if l.strip())[1:]   # drop the slowest (warm-up)
+        t = float(np.median(us)) * 1e-6
+        kind = "reference"
+        sample = (f"whole {w}x{h} pair, {nwin} windows, reference FP64 algos::FFT path + ThreadPool "
+                  f"'pool' strategy (examples/process/main.cpp:189-276), median of {len(us)} timed loops")
+    else:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as orc
+        t0 = time.perf_counter()
+        orc.process_pair(a, b, win, step, step, threads=threads)
+        t = time.perf_counter() - t0
+        kind = "port"
+        sample = f"whole {w}x{h} pair, {nwin} windows, FP64 oracle restatement, one timed loop"
+    return {"value": nwin / t, "unit": "vectors/s", "cores": threads, "kind": kind, "sample": sample}
+
+
+def accuracy(res, a, b, win, step, sample=4096, seed=0):
+    """Max |d(u,v)| against the FP64 oracle on a bounded window sample."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    import openpiv_amd as oa
+    n = len(res)
+    rng = np.random.default_rng(seed)
+    first = int(rng.integers(0, max(1, n - sample)))
+    ref = orc.process_pair(a, b, win, step, step, threads=min(16, os.cpu_count() or 1), first=first, count=sample)
+    got = res[first:first + len(ref)]
+    tie = (got["flags"] & (oa.FLAG_NEAR_TIE | oa.FLAG_EXACT_TIE | oa.FLAG_FEWER_THAN_2_PEAKS)) != 0
+    ok = ~tie & np.isfinite(ref["u"]) & np.isfinite(ref["v"])
+    mism = int(((got["px"] != ref["px"]) | (got["py"] != ref["py"]))[ok].sum())
+    err = float(max(np.abs(got["u"][ok] - ref["u"][ok]).max(), np.abs(got["v"][ok] - ref["v"][ok]).max()))
+    return {"max_err_px": err, "checked_windows": int(len(ref)), "peak_mismatches": mism,
+            "near_ties": int(tie.sum())}
+
+
+def load_traffic(key: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, copy-kernel calibrated), committed under profiles/."""
+    f = ROOT / "profiles" / "traffic.json"
+    if f.exists():
+        d = json.loads(f.read_text())
+        if key in d:
+            return d[key]
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--win", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
+              "(multi-GPU sharding is a design note, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import openpiv_amd as oa
+
+    win = args.win
+    log(f"[bench] generating 4096x4096 f32 pair, win={win}")
+    a, b, w, step = workload(win)
+    ctx = oa.Context(0)
+    plan = oa.Plan(ctx, w, w, win, step=(step, step), dtype=np.float32)
+    n = plan.count
+    da = torch.from_numpy(a).to("cuda:0")
+    db = torch.from_numpy(b).to("cuda:0")
+    dout = torch.zeros(n * RESULT_BYTES, dtype=torch.uint8, device="cuda:0")
+    torch.cuda.synchronize()
+
+    ev = HipEvents()
+    stream = ctx.stream
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        plan.run_async(da.data_ptr(), db.data_ptr(), dout.data_ptr())
+    ctx.synchronize()
+    torch.cuda.synchronize()
+
+    log(f"[bench] timing {args.steps} steps")
+    e0, e1 = ev.create(), ev.create()
+    t0 = time.perf_counter()
+    ev.record(e0, stream)
+    for _ in range(args.steps):
+        plan.run_async(da.data_ptr(), db.data_ptr(), dout.data_ptr())
+    ev.record(e1, stream)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev.elapsed_ms(e0, e1)
+    ms_per_step = ms / args.steps
+    value = n / (ms_per_step * 1e-3)
+
+    res = dout.cpu().numpy().view(oa.RESULT_DTYPE)
+    log("[bench] accuracy sample vs FP64 oracle")
+    acc = accuracy(res, a, b, win, step)
+
+    frame_bytes = 2 * w * w * 4
+    alg_bytes = frame_bytes + n * RESULT_BYTES
+    achieved = alg_bytes / (ms_per_step * 1e-3) / 1e9
+    key = f"fft_fp32_w{win}_4096_f32"
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(key),
+                "algorithmic_bytes_per_launch": alg_bytes,
+                "bytes_per_window": round(alg_bytes / n, 1)}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(16, (os.cpu_count() or 2)) - 1)
+        log(f"[bench] cpu baseline, {threads} threads")
+        cpu = cpu_baseline(a, b, win, step, threads)
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "vectors/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"4096x4096 f32 frame pair, rotation |d|max 6 px, {win}x{win} windows "
+                               f"step {step} ({n} windows), fused FP32 FFT correlation + find_peaks + "
+                               f"Gaussian fit", "windows": n, "window": win, "step": step,
+                   "path": "fft_fp32_fused", "frames_resident_in_hbm": True},
+        "roofline": roofline, "cpu_baseline": cpu, "accuracy": acc,
+        "wall_s_timed_region": round(wall, 4),
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,149 @@
+/*
+ * openpiv_amd.h -- C ABI of the MI355X (gfx950) PIV window-correlation engine.
+ *
+ * This is the drop-in boundary for the hot path of openpiv-cxx
+ * (ErichZimmer/openpiv-cxx 0.2.0): per-window FFT circular cross-correlation,
+ * find_peaks(.., 2, 1) and the 3-point Gaussian fit, run over the windows of
+ * generate_cartesian_grid, as examples/process/main.cpp:189-302 drives them.
+ * The reference exposes that path as C++ templates; each entry point below
+ * names the reference interface it replaces.
+ *
+ * Conventions: every function returns an opiv_status (0 = OPIV_OK); no C++
+ * exception crosses this boundary.  On failure opiv_last_error() returns the
+ * message of the failing call on the calling thread; messages that mirror a
+ * reference exception carry the reference's text so a C++ wrapper can rethrow
+ * it verbatim.  Frame pointers may be host or device memory.
+ */
+#ifndef OPENPIV_AMD_H
+#define OPENPIV_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum opiv_status {
+    OPIV_OK = 0,
+    OPIV_E_INVALID = 1,      /* bad argument (null pointer, zero size, bad enum) */
+    OPIV_E_NOT_POW2 = 2,     /* fft.h:82-83 "dimensions must be power of 2: ..." */
+    OPIV_E_SIZE = 3,         /* fft.h:94-98 "image size is different from expected: ..." */
+    OPIV_E_GRID = 4,         /* grid.impl.h:14-40 grid errors */
+    OPIV_E_HIP = 5,          /* a HIP runtime call failed */
+    OPIV_E_UNSUPPORTED = 6,  /* configuration outside the engine's range */
+    OPIV_E_NO_DEVICE = 7     /* no usable gfx950 device */
+} opiv_status;
+
+typedef enum opiv_dtype {   /* pixel types of the frames (loaders deliver g_8 / g_16) */
+    OPIV_U8 = 0,
+    OPIV_U16 = 1,
+    OPIV_F32 = 2,
+    OPIV_F64 = 3             /* core::gf_image; widened pixels are narrowed to FP32 on load */
+} opiv_dtype;
+
+typedef enum opiv_variant {
+    /* exact circular cross-correlation: algos::FFT::cross_correlate
+     * (fft.h:188-206); pocket / pocket_real (pocket_fft.h:236-275) compute
+     * the same correlation.  FP32 arithmetic, FP64 Gaussian fit. */
+    OPIV_CIRCULAR_FFT_FP32 = 0,
+    /* algos::FFT::cross_correlate_real (fft.h:208-227) including its
+     * unpack (fft.h:166-183) that leaves spectral row 0, row H/2 and
+     * column 0 zero */
+    OPIV_REAL_COMPAT = 1
+} opiv_variant;
+
+typedef enum opiv_flag {
+    OPIV_FLAG_FEWER_THAN_2_PEAKS = 1u << 0, /* main.cpp:214-218: record left all-zero */
+    OPIV_FLAG_NONFINITE_FIT = 1u << 1,      /* a stencil value <= 0 made the fit NaN/inf */
+    OPIV_FLAG_NEAR_TIE = 1u << 2,           /* top-2 (or peak vs neighbour) closer than the FP32 tolerance */
+    OPIV_FLAG_EXACT_TIE = 1u << 3           /* top-2 values equal (reference order is std::sort's) */
+} opiv_flag;
+
+/* One vector of examples/process (main.cpp:144-150 point_vector: xy, vxy,
+ * sn) plus the integer peak, both peak heights and flags. */
+typedef struct opiv_window_result {
+    double x, y;       /* window midpoint; y = image height - mid_y (main.cpp:226-230) */
+    double u, v;       /* mid - (bottom_left + fitted peak) (main.cpp:227) */
+    double sn;         /* peak0 / peak1 if peak1 > 0, else 0 (main.cpp:233-234) */
+    double peak0;      /* correlation value at the integer peak */
+    double peak1;      /* second highest local maximum */
+    int32_t px, py;    /* integer peak in the quadrant-swapped plane */
+    uint32_t flags;    /* opiv_flag bits */
+    uint32_t reserved;
+} opiv_window_result;
+
+typedef struct opiv_ctx opiv_ctx;
+typedef struct opiv_plan opiv_plan;
+
+/* Plan for one frame size / window size / grid; the batched replacement of
+ * `static algos::FFT fft{ia}` + the grid of main.cpp:137-138. */
+typedef struct opiv_plan_desc {
+    uint32_t img_w, img_h;   /* frame size in pixels */
+    size_t pitch;            /* elements per frame row (0 = img_w) */
+    int32_t dtype;           /* opiv_dtype */
+    uint32_t win;            /* square window side, power of 2 in [8, 64] */
+    uint32_t step_x, step_y; /* grid offsets in pixels (grid.impl.h:24-61); 0 = use overlap */
+    double overlap;          /* process -o: step = (uint32)(win * overlap) (grid.impl.h:10-22) */
+    int32_t variant;         /* opiv_variant */
+    int32_t limit_search;    /* process -l: search the dilate(0.5) centre (main.cpp:204-211) */
+} opiv_plan_desc;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* opiv_last_error(void);
+
+/* Library version string. */
+const char* opiv_version(void);
+
+/* Context: one device, one HIP stream.  A context must not be used from two
+ * host threads at once without external locking. */
+int opiv_ctx_create(int device, opiv_ctx** out);
+int opiv_ctx_destroy(opiv_ctx* ctx);
+int opiv_ctx_synchronize(opiv_ctx* ctx);
+/* The hipStream_t every launch of this context is enqueued on. */
+void* opiv_ctx_stream(opiv_ctx* ctx);
+
+/* core::generate_cartesian_grid (grid.impl.h:10-61), host only.  Writes
+ * bottom-left (x, y) pairs in the reference's y-outer order when xy != NULL. */
+int opiv_grid_offsets(uint32_t win_w, uint32_t win_h, double overlap,
+                      uint32_t* off_x, uint32_t* off_y);
+int opiv_grid(uint32_t img_w, uint32_t img_h, uint32_t win_w, uint32_t win_h,
+              uint32_t off_x, uint32_t off_y, uint32_t* nx, uint32_t* ny, int32_t* xy);
+
+/* Plan creation checks the window the way algos::FFT's constructor does
+ * (fft.h:76-84) and builds the grid (main.cpp:137-138). */
+int opiv_plan_create(opiv_ctx* ctx, const opiv_plan_desc* desc, opiv_plan** out);
+int opiv_plan_destroy(opiv_plan* plan);
+int opiv_plan_window_count(const opiv_plan* plan, size_t* count);
+int opiv_plan_grid(const opiv_plan* plan, int32_t* xy);
+
+/* The whole processing loop of examples/process/main.cpp:241-302 for one
+ * frame pair: every window's correlation, find_peaks(.., 2, 1),
+ * fit_simple_gaussian and vector, in grid order.  a, b, out may be host or
+ * device pointers; n must equal the plan's window count.  Returns when the
+ * results are in `out`. */
+int opiv_run_pair(opiv_plan* plan, const void* a, const void* b,
+                  opiv_window_result* out, size_t n);
+
+/* Same, device pointers only, enqueued on the context stream without
+ * synchronising (the caller synchronises). */
+int opiv_run_pair_async(opiv_plan* plan, const void* d_a, const void* d_b,
+                        opiv_window_result* d_out, size_t n);
+
+/* correlator_t of main.cpp:153 for a batch of n windows:
+ * FFT::cross_correlate (fft.h:188-206) or cross_correlate_real
+ * (fft.h:208-227) of stacked win*win FP64 windows a[i], b[i]; writes the
+ * quadrant-swapped real planes (FP64, computed in FP32).  Host or device
+ * pointers. */
+int opiv_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t variant,
+                           const double* a, const double* b, double* planes);
+
+/* FFT::auto_correlate (fft.h:229-242) for n stacked windows (real part of
+ * the returned plane). */
+int opiv_auto_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n,
+                                const double* a, double* planes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

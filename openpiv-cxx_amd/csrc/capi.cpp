@@ -1,0 +1,400 @@
+// capi.cpp -- the C ABI of include/openpiv_amd.h over the HIP kernels.
+//
+// Host-side replacement of the per-window loop of
+// examples/process/main.cpp:137-302: the grid is generated once per plan
+// (grid.impl.h:10-61), and one kernel launch processes every window of a
+// frame pair.  Error texts that mirror reference exceptions use the
+// reference's wording.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "corr_fft.hpp"
+#include "openpiv_amd.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(OPIV_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(call)                                        \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);    \
+    } while (0)
+
+bool is_pow2(uint64_t v) { return v && (v == (v & (~v + 1))); }   // util.h:67-70
+
+size_t dtype_size(int dt)
+{
+    switch (dt) {
+    case OPIV_U8: return 1;
+    case OPIV_U16: return 2;
+    case OPIV_F32: return 4;
+    case OPIV_F64: return 8;
+    default: return 0;
+    }
+}
+
+// default relative tolerance under which two FP32 correlation values are
+// treated as tied for OPIV_FLAG_NEAR_TIE
+constexpr float kTieTol = 2e-6f;
+
+// device buffer that grows on demand
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+bool is_device_ptr(const void* ptr)
+{
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof(attr));
+    hipError_t e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();   // unregistered host memory: clear the sticky error
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+} // namespace
+
+struct opiv_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf stage_a, stage_b, stage_out;
+};
+
+struct opiv_plan {
+    opiv_ctx* ctx = nullptr;
+    opiv_plan_desc desc{};
+    uint32_t off_x = 0, off_y = 0;
+    uint32_t nx = 0, ny = 0;
+    int32_t x_start = 0, y_start = 0;
+};
+
+extern "C" {
+
+const char* opiv_last_error(void) { return g_last_error.c_str(); }
+
+const char* opiv_version(void) { return "openpiv-amd 0.1.0 (gfx950)"; }
+
+int opiv_ctx_create(int device, opiv_ctx** out)
+{
+    if (!out) return fail(OPIV_E_INVALID, "opiv_ctx_create: null output pointer");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return fail(OPIV_E_NO_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= count)
+        return fail(OPIV_E_INVALID, "device %d out of range (%d devices)", device, count);
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(OPIV_E_NO_DEVICE, "device %d is %s; this library is built for gfx950 only", device, prop.gcnArchName);
+    HIP_TRY(hipSetDevice(device));
+    opiv_ctx* c = new opiv_ctx;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    *out = c;
+    return OPIV_OK;
+}
+
+int opiv_ctx_destroy(opiv_ctx* ctx)
+{
+    if (!ctx) return OPIV_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->stage_a.release();
+    ctx->stage_b.release();
+    ctx->stage_out.release();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return OPIV_OK;
+}
+
+int opiv_ctx_synchronize(opiv_ctx* ctx)
+{
+    if (!ctx) return fail(OPIV_E_INVALID, "opiv_ctx_synchronize: null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return OPIV_OK;
+}
+
+void* opiv_ctx_stream(opiv_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int opiv_grid_offsets(uint32_t win_w, uint32_t win_h, double overlap, uint32_t* off_x, uint32_t* off_y)
+{
+    if (!off_x || !off_y) return fail(OPIV_E_INVALID, "opiv_grid_offsets: null output pointer");
+    // grid.impl.h:14-21
+    if (overlap < 0.0 || overlap > 1.0) return fail(OPIV_E_GRID, "offsets must be between 0.0 and 1.0");
+    *off_x = (uint32_t)(win_w * overlap);
+    *off_y = (uint32_t)(win_h * overlap);
+    return OPIV_OK;
+}
+
+int opiv_grid(uint32_t img_w, uint32_t img_h, uint32_t win_w, uint32_t win_h,
+              uint32_t off_x, uint32_t off_y, uint32_t* nx, uint32_t* ny, int32_t* xy)
+{
+    if (!nx || !ny) return fail(OPIV_E_INVALID, "opiv_grid: null output pointer");
+    // grid.impl.h:29-40 (size::area() is a uint32 product)
+    if ((uint32_t)(img_w * img_h) == 0) return fail(OPIV_E_GRID, "image size must be non-zero");
+    if ((uint32_t)(win_w * win_h) == 0) return fail(OPIV_E_GRID, "interrogation size must be non-zero");
+    if (off_x == 0 || off_y == 0) return fail(OPIV_E_GRID, "offsets must be non-zero");
+    if (win_w > img_w || win_h > img_h) return fail(OPIV_E_GRID, "interrogation size is bigger than image");
+    // grid.impl.h:44-58
+    const uint32_t x_count = 1 + (img_w - win_w) / off_x;
+    const uint32_t y_count = 1 + (img_h - win_h) / off_y;
+    const uint32_t x_start = (img_w - win_w - off_x * (x_count - 1)) / 2;
+    const uint32_t y_start = (img_h - win_h - off_y * (y_count - 1)) / 2;
+    *nx = x_count;
+    *ny = y_count;
+    if (xy) {
+        size_t k = 0;
+        for (uint32_t y = 0; y < y_count; ++y)
+            for (uint32_t x = 0; x < x_count; ++x, ++k) {
+                xy[2 * k] = (int32_t)(x_start + x * off_x);
+                xy[2 * k + 1] = (int32_t)(y_start + y * off_y);
+            }
+    }
+    return OPIV_OK;
+}
+
+int opiv_plan_create(opiv_ctx* ctx, const opiv_plan_desc* desc, opiv_plan** out)
+{
+    if (!ctx || !desc || !out) return fail(OPIV_E_INVALID, "opiv_plan_create: null argument");
+    *out = nullptr;
+    const uint32_t win = desc->win;
+    // fft.h:82-83
+    if (!is_pow2(win)) return fail(OPIV_E_NOT_POW2, "dimensions must be power of 2: [%u,%u]", win, win);
+    if (win < 8 || win > 64)
+        return fail(OPIV_E_UNSUPPORTED, "window %u outside the supported range [8, 64]", win);
+    if (dtype_size(desc->dtype) == 0) return fail(OPIV_E_INVALID, "unknown dtype %d", desc->dtype);
+    if (desc->variant != OPIV_CIRCULAR_FFT_FP32 && desc->variant != OPIV_REAL_COMPAT)
+        return fail(OPIV_E_UNSUPPORTED, "unknown variant %d", desc->variant);
+    const size_t pitch = desc->pitch ? desc->pitch : desc->img_w;
+    if (pitch < desc->img_w) return fail(OPIV_E_INVALID, "pitch %zu smaller than width %u", pitch, desc->img_w);
+
+    opiv_plan* p = new opiv_plan;
+    p->ctx = ctx;
+    p->desc = *desc;
+    p->desc.pitch = pitch;
+    int rc = OPIV_OK;
+    if (desc->step_x && desc->step_y) {
+        p->off_x = desc->step_x;
+        p->off_y = desc->step_y;
+    } else {
+        rc = opiv_grid_offsets(win, win, desc->overlap, &p->off_x, &p->off_y);
+    }
+    if (rc == OPIV_OK)
+        rc = opiv_grid(desc->img_w, desc->img_h, win, win, p->off_x, p->off_y, &p->nx, &p->ny, nullptr);
+    if (rc != OPIV_OK) {
+        delete p;
+        return rc;
+    }
+    if ((uint64_t)p->nx * p->ny > 0x7fffffffull) {
+        delete p;
+        return fail(OPIV_E_UNSUPPORTED, "too many windows");
+    }
+    p->x_start = (int32_t)((desc->img_w - win - p->off_x * (p->nx - 1)) / 2);
+    p->y_start = (int32_t)((desc->img_h - win - p->off_y * (p->ny - 1)) / 2);
+    *out = p;
+    return OPIV_OK;
+}
+
+int opiv_plan_destroy(opiv_plan* plan)
+{
+    delete plan;
+    return OPIV_OK;
+}
+
+int opiv_plan_window_count(const opiv_plan* plan, size_t* count)
+{
+    if (!plan || !count) return fail(OPIV_E_INVALID, "opiv_plan_window_count: null argument");
+    *count = (size_t)plan->nx * plan->ny;
+    return OPIV_OK;
+}
+
+int opiv_plan_grid(const opiv_plan* plan, int32_t* xy)
+{
+    if (!plan || !xy) return fail(OPIV_E_INVALID, "opiv_plan_grid: null argument");
+    uint32_t nx, ny;
+    return opiv_grid(plan->desc.img_w, plan->desc.img_h, plan->desc.win, plan->desc.win,
+                     plan->off_x, plan->off_y, &nx, &ny, xy);
+}
+
+static opiv::CorrParams plan_params(const opiv_plan* plan, const void* a, const void* b, opiv_window_result* out)
+{
+    opiv::CorrParams p{};
+    p.a = a;
+    p.b = b;
+    p.pitch = plan->desc.pitch;
+    p.x_start = plan->x_start;
+    p.y_start = plan->y_start;
+    p.step_x = (int32_t)plan->off_x;
+    p.step_y = (int32_t)plan->off_y;
+    p.nx = (int32_t)plan->nx;
+    p.first = 0;
+    p.n_windows = (int32_t)(plan->nx * plan->ny);
+    p.img_h = plan->desc.img_h;
+    p.limit_search = plan->desc.limit_search ? 1 : 0;
+    p.real_compat = plan->desc.variant == OPIV_REAL_COMPAT ? 1 : 0;
+    p.tie_tol = kTieTol;
+    p.out = out;
+    p.planes = nullptr;
+    return p;
+}
+
+int opiv_run_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, opiv_window_result* d_out, size_t n)
+{
+    if (!plan || !d_a || !d_b || !d_out) return fail(OPIV_E_INVALID, "opiv_run_pair_async: null argument");
+    const size_t count = (size_t)plan->nx * plan->ny;
+    if (n != count) return fail(OPIV_E_SIZE, "result count %zu differs from the grid's %zu windows", n, count);
+    HIP_TRY(hipSetDevice(plan->ctx->device));
+    opiv::CorrParams p = plan_params(plan, d_a, d_b, d_out);
+    HIP_TRY(opiv::launch_corr_fft(p, (int)plan->desc.win, plan->desc.dtype, false, plan->ctx->stream));
+    return OPIV_OK;
+}
+
+int opiv_run_pair(opiv_plan* plan, const void* a, const void* b, opiv_window_result* out, size_t n)
+{
+    if (!plan || !a || !b || !out) return fail(OPIV_E_INVALID, "opiv_run_pair: null argument");
+    opiv_ctx* ctx = plan->ctx;
+    const size_t count = (size_t)plan->nx * plan->ny;
+    if (n != count) return fail(OPIV_E_SIZE, "result count %zu differs from the grid's %zu windows", n, count);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t frame_bytes = plan->desc.pitch * plan->desc.img_h * dtype_size(plan->desc.dtype);
+    const void* da = a;
+    const void* db = b;
+    if (!is_device_ptr(a)) {
+        HIP_TRY(ctx->stage_a.ensure(frame_bytes));
+        HIP_TRY(hipMemcpyAsync(ctx->stage_a.p, a, frame_bytes, hipMemcpyHostToDevice, ctx->stream));
+        da = ctx->stage_a.p;
+    }
+    if (!is_device_ptr(b)) {
+        HIP_TRY(ctx->stage_b.ensure(frame_bytes));
+        HIP_TRY(hipMemcpyAsync(ctx->stage_b.p, b, frame_bytes, hipMemcpyHostToDevice, ctx->stream));
+        db = ctx->stage_b.p;
+    }
+    const bool out_dev = is_device_ptr(out);
+    opiv_window_result* dout = out;
+    if (!out_dev) {
+        HIP_TRY(ctx->stage_out.ensure(count * sizeof(opiv_window_result)));
+        dout = (opiv_window_result*)ctx->stage_out.p;
+    }
+    opiv::CorrParams p = plan_params(plan, da, db, dout);
+    HIP_TRY(opiv::launch_corr_fft(p, (int)plan->desc.win, plan->desc.dtype, false, ctx->stream));
+    if (!out_dev)
+        HIP_TRY(hipMemcpyAsync(out, dout, count * sizeof(opiv_window_result), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return OPIV_OK;
+}
+
+static int correlate_stacked(opiv_ctx* ctx, uint32_t win, size_t n, int variant,
+                             const double* a, const double* b, double* planes)
+{
+    if (!ctx || !a || !b || !planes) return fail(OPIV_E_INVALID, "correlate: null argument");
+    if (!is_pow2(win)) return fail(OPIV_E_NOT_POW2, "dimensions must be power of 2: [%u,%u]", win, win);
+    if (win < 8 || win > 64) return fail(OPIV_E_UNSUPPORTED, "window %u outside the supported range [8, 64]", win);
+    if (variant != OPIV_CIRCULAR_FFT_FP32 && variant != OPIV_REAL_COMPAT)
+        return fail(OPIV_E_UNSUPPORTED, "unknown variant %d", variant);
+    if (n == 0) return OPIV_OK;
+    if (n > 0x7fffffffull / win) return fail(OPIV_E_UNSUPPORTED, "too many windows");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t bytes = n * win * win * sizeof(double);
+    const void* da = a;
+    const void* db = b;
+    if (!is_device_ptr(a)) {
+        HIP_TRY(ctx->stage_a.ensure(bytes));
+        HIP_TRY(hipMemcpyAsync(ctx->stage_a.p, a, bytes, hipMemcpyHostToDevice, ctx->stream));
+        da = ctx->stage_a.p;
+    }
+    if (b == a) {
+        db = da;
+    } else if (!is_device_ptr(b)) {
+        HIP_TRY(ctx->stage_b.ensure(bytes));
+        HIP_TRY(hipMemcpyAsync(ctx->stage_b.p, b, bytes, hipMemcpyHostToDevice, ctx->stream));
+        db = ctx->stage_b.p;
+    }
+    const bool out_dev = is_device_ptr(planes);
+    double* dplanes = planes;
+    if (!out_dev) {
+        HIP_TRY(ctx->stage_out.ensure(bytes));
+        dplanes = (double*)ctx->stage_out.p;
+    }
+    // the stacked windows form an image win wide and n*win high with a
+    // one-column grid of step win
+    opiv::CorrParams p{};
+    p.a = da;
+    p.b = db;
+    p.pitch = win;
+    p.x_start = 0;
+    p.y_start = 0;
+    p.step_x = (int32_t)win;
+    p.step_y = (int32_t)win;
+    p.nx = 1;
+    p.first = 0;
+    p.n_windows = (int32_t)n;
+    p.img_h = (uint32_t)(n * win);
+    p.real_compat = variant == OPIV_REAL_COMPAT;
+    p.tie_tol = kTieTol;
+    p.planes = dplanes;
+    HIP_TRY(opiv::launch_corr_fft(p, (int)win, OPIV_F64, true, ctx->stream));
+    if (!out_dev) HIP_TRY(hipMemcpyAsync(planes, dplanes, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return OPIV_OK;
+}
+
+int opiv_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t variant,
+                           const double* a, const double* b, double* planes)
+{
+    return correlate_stacked(ctx, win, n, variant, a, b, planes);
+}
+
+int opiv_auto_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n, const double* a, double* planes)
+{
+    // FFT(a) conj(FFT(a)) = |FFT(a)|^2: the cross-correlation of a with itself
+    return correlate_stacked(ctx, win, n, OPIV_CIRCULAR_FFT_FP32, a, a, planes);
+}
+
+} // extern "C"

@@ -1,0 +1,321 @@
+"""Python bindings (ctypes) of the openpiv-amd C ABI (include/openpiv_amd.h).
+
+The engine itself is the HIP library ``lib/libopenpiv_amd.so``; this module
+only marshals arguments.  There is no CPU fallback: every compute call goes
+through the gfx950 kernels, and loading fails loudly when the library has not
+been built.
+
+Reference interface mirrored (ErichZimmer/openpiv-cxx 0.2.0):
+  * ``grid`` / ``grid_offsets``   -> core::generate_cartesian_grid (core/detail/grid.impl.h:10-61)
+  * ``Plan.run``                  -> the per-window loop of examples/process/main.cpp:189-302
+  * ``Context.cross_correlate``   -> algos::FFT::cross_correlate / cross_correlate_real (algos/fft.h:188-227)
+  * ``Context.auto_correlate``    -> algos::FFT::auto_correlate (algos/fft.h:229-242)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_DIR = PKG_ROOT / "lib"
+LIB_PATH = LIB_DIR / "libopenpiv_amd.so"
+SYNTH_PATH = LIB_DIR / "libopiv_synth.so"
+
+# opiv_dtype
+U8, U16, F32, F64 = 0, 1, 2, 3
+DTYPE_CODES = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16,
+               np.dtype(np.float32): F32, np.dtype(np.float64): F64}
+# opiv_variant
+CIRCULAR_FFT_FP32 = 0
+REAL_COMPAT = 1
+# opiv_flag
+FLAG_FEWER_THAN_2_PEAKS = 1
+FLAG_NONFINITE_FIT = 2
+FLAG_NEAR_TIE = 4
+FLAG_EXACT_TIE = 8
+
+RESULT_DTYPE = np.dtype([
+    ("x", "<f8"), ("y", "<f8"), ("u", "<f8"), ("v", "<f8"), ("sn", "<f8"),
+    ("peak0", "<f8"), ("peak1", "<f8"),
+    ("px", "<i4"), ("py", "<i4"), ("flags", "<u4"), ("reserved", "<u4"),
+])
+assert RESULT_DTYPE.itemsize == 72
+
+# every symbol include/openpiv_amd.h declares
+EXPORTS = (
+    "opiv_last_error", "opiv_version",
+    "opiv_ctx_create", "opiv_ctx_destroy", "opiv_ctx_synchronize", "opiv_ctx_stream",
+    "opiv_grid_offsets", "opiv_grid",
+    "opiv_plan_create", "opiv_plan_destroy", "opiv_plan_window_count", "opiv_plan_grid",
+    "opiv_run_pair", "opiv_run_pair_async",
+    "opiv_correlate_windows", "opiv_auto_correlate_windows",
+)
+
+
+class OpivError(RuntimeError):
+    """A non-zero opiv_status; ``code`` holds it, the message is opiv_last_error()."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("img_w", ctypes.c_uint32), ("img_h", ctypes.c_uint32),
+        ("pitch", ctypes.c_size_t), ("dtype", ctypes.c_int32),
+        ("win", ctypes.c_uint32),
+        ("step_x", ctypes.c_uint32), ("step_y", ctypes.c_uint32),
+        ("overlap", ctypes.c_double),
+        ("variant", ctypes.c_int32), ("limit_search", ctypes.c_int32),
+    ]
+
+
+class SynthDesc(ctypes.Structure):
+    _fields_ = [
+        ("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+        ("density", ctypes.c_double), ("diameter", ctypes.c_double),
+        ("i0_lo", ctypes.c_double), ("i0_hi", ctypes.c_double),
+        ("noise_sigma", ctypes.c_double), ("field", ctypes.c_int),
+        ("p0", ctypes.c_double), ("p1", ctypes.c_double),
+        ("p2", ctypes.c_double), ("p3", ctypes.c_double),
+        ("dtype", ctypes.c_int), ("nthreads", ctypes.c_int),
+    ]
+
+
+_lib = None
+_synth = None
+
+
+def library_path() -> Path:
+    return LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP engine library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise OSError(f"{LIB_PATH} is missing: build it with `make -C {PKG_ROOT}` "
+                      "(or __graft_entry__.build())")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp, sz, u32, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int32
+    lib.opiv_last_error.restype = ctypes.c_char_p
+    lib.opiv_version.restype = ctypes.c_char_p
+    lib.opiv_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    lib.opiv_ctx_destroy.argtypes = [vp]
+    lib.opiv_ctx_synchronize.argtypes = [vp]
+    lib.opiv_ctx_stream.argtypes = [vp]
+    lib.opiv_ctx_stream.restype = vp
+    lib.opiv_grid_offsets.argtypes = [u32, u32, ctypes.c_double, ctypes.POINTER(u32), ctypes.POINTER(u32)]
+    lib.opiv_grid.argtypes = [u32, u32, u32, u32, u32, u32, ctypes.POINTER(u32), ctypes.POINTER(u32), vp]
+    lib.opiv_plan_create.argtypes = [vp, ctypes.POINTER(PlanDesc), ctypes.POINTER(vp)]
+    lib.opiv_plan_destroy.argtypes = [vp]
+    lib.opiv_plan_window_count.argtypes = [vp, ctypes.POINTER(sz)]
+    lib.opiv_plan_grid.argtypes = [vp, vp]
+    lib.opiv_run_pair.argtypes = [vp, vp, vp, vp, sz]
+    lib.opiv_run_pair_async.argtypes = [vp, vp, vp, vp, sz]
+    lib.opiv_correlate_windows.argtypes = [vp, u32, sz, i32, vp, vp, vp]
+    lib.opiv_auto_correlate_windows.argtypes = [vp, u32, sz, vp, vp]
+    for name in EXPORTS:
+        if name not in ("opiv_last_error", "opiv_version", "opiv_ctx_stream"):
+            getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = load().opiv_last_error().decode()
+        raise OpivError(rc, msg)
+
+
+def version() -> str:
+    return load().opiv_version().decode()
+
+
+def grid_offsets(win_w: int, win_h: int, overlap: float) -> tuple[int, int]:
+    """generate_cartesian_grid's step from an overlap fraction (grid.impl.h:10-22)."""
+    ox, oy = ctypes.c_uint32(), ctypes.c_uint32()
+    _check(load().opiv_grid_offsets(win_w, win_h, overlap, ctypes.byref(ox), ctypes.byref(oy)))
+    return ox.value, oy.value
+
+
+def grid(img_w: int, img_h: int, win_w: int, win_h: int, off_x: int, off_y: int) -> np.ndarray:
+    """Bottom-left corners, shape (n, 2) int32, in the reference's y-outer order."""
+    lib = load()
+    nx, ny = ctypes.c_uint32(), ctypes.c_uint32()
+    _check(lib.opiv_grid(img_w, img_h, win_w, win_h, off_x, off_y, ctypes.byref(nx), ctypes.byref(ny), None))
+    xy = np.zeros((nx.value * ny.value, 2), dtype=np.int32)
+    _check(lib.opiv_grid(img_w, img_h, win_w, win_h, off_x, off_y, ctypes.byref(nx), ctypes.byref(ny),
+                         xy.ctypes.data))
+    return xy
+
+
+def _ptr(x) -> int:
+    """Data pointer of a numpy array (host) or a torch tensor (host or device)."""
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    if hasattr(x, "data_ptr"):
+        return int(x.data_ptr())
+    if isinstance(x, int):
+        return x
+    raise TypeError(f"unsupported buffer type {type(x)!r}")
+
+
+class Context:
+    """One device and one HIP stream (opiv_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        _check(self._lib.opiv_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.opiv_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return self._lib.opiv_ctx_stream(self.handle) or 0
+
+    def synchronize(self) -> None:
+        _check(self._lib.opiv_ctx_synchronize(self.handle))
+
+    def cross_correlate(self, a: np.ndarray, b: np.ndarray, variant: int = CIRCULAR_FFT_FP32) -> np.ndarray:
+        """FFT::cross_correlate(_real) of stacked square windows (n, W, W) -> swapped planes."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if a.ndim == 2:
+            a, b = a[None], b[None]
+        if a.shape != b.shape or a.shape[1] != a.shape[2]:
+            raise OpivError(3, f"image size is different from expected: {a.shape}, {b.shape}")
+        n, win = a.shape[0], a.shape[1]
+        out = np.empty_like(a)
+        _check(self._lib.opiv_correlate_windows(self.handle, win, n, variant, a.ctypes.data, b.ctypes.data,
+                                                out.ctypes.data))
+        return out
+
+    def auto_correlate(self, a: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.ndim == 2:
+            a = a[None]
+        n, win = a.shape[0], a.shape[1]
+        out = np.empty_like(a)
+        _check(self._lib.opiv_auto_correlate_windows(self.handle, win, n, a.ctypes.data, out.ctypes.data))
+        return out
+
+
+class Plan:
+    """A frame size + window + grid (opiv_plan); ``run`` processes one pair."""
+
+    def __init__(self, ctx: Context, img_w: int, img_h: int, win: int, *, overlap: float = 0.5,
+                 step: tuple[int, int] | None = None, dtype=np.float32, pitch: int = 0,
+                 variant: int = CIRCULAR_FFT_FP32, limit_search: bool = False):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.dtype = np.dtype(dtype)
+        d = PlanDesc()
+        d.img_w, d.img_h, d.pitch = img_w, img_h, pitch
+        d.dtype = DTYPE_CODES[self.dtype]
+        d.win = win
+        d.step_x, d.step_y = step if step else (0, 0)
+        d.overlap = overlap
+        d.variant = variant
+        d.limit_search = 1 if limit_search else 0
+        self.desc = d
+        h = ctypes.c_void_p()
+        _check(self._lib.opiv_plan_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)))
+        self.handle = h
+        n = ctypes.c_size_t()
+        _check(self._lib.opiv_plan_window_count(h, ctypes.byref(n)))
+        self.count = n.value
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.opiv_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid(self) -> np.ndarray:
+        xy = np.zeros((self.count, 2), dtype=np.int32)
+        _check(self._lib.opiv_plan_grid(self.handle, xy.ctypes.data))
+        return xy
+
+    def run(self, a, b, out=None) -> np.ndarray:
+        """Process one frame pair (host numpy arrays or device tensors); returns results."""
+        if isinstance(a, np.ndarray):
+            a = np.ascontiguousarray(a, dtype=self.dtype)
+            b = np.ascontiguousarray(b, dtype=self.dtype)
+        if out is None:
+            out = np.zeros(self.count, dtype=RESULT_DTYPE)
+        _check(self._lib.opiv_run_pair(self.handle, _ptr(a), _ptr(b), _ptr(out), self.count))
+        return out
+
+    def run_async(self, a_ptr: int, b_ptr: int, out_ptr: int) -> None:
+        """Enqueue on the context stream; device pointers only."""
+        _check(self._lib.opiv_run_pair_async(self.handle, a_ptr, b_ptr, out_ptr, self.count))
+
+
+# ---------------------------------------------------------------- synthetic frames
+
+FIELD_UNIFORM, FIELD_ROTATION, FIELD_LAMB_OSEEN = 0, 1, 2
+_SYNTH_CODES = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1,
+                np.dtype(np.float32): 2, np.dtype(np.float64): 3}
+
+
+def _load_synth():
+    global _synth
+    if _synth is None:
+        if not SYNTH_PATH.exists():
+            raise OSError(f"{SYNTH_PATH} is missing: build it with `make -C {PKG_ROOT}`")
+        _synth = ctypes.CDLL(str(SYNTH_PATH))
+        _synth.opiv_synth_pair.argtypes = [ctypes.POINTER(SynthDesc), ctypes.c_void_p, ctypes.c_void_p]
+        _synth.opiv_synth_pair.restype = ctypes.c_int
+    return _synth
+
+
+def synth_pair(w: int, h: int, *, seed: int = 1, density: float = 0.05, diameter: float = 2.5,
+               i0=(100.0, 255.0), noise: float = 2.0, field: int = FIELD_UNIFORM,
+               params=(0.0, 0.0, 0.0, 0.0), dtype=np.float64, threads: int = 0):
+    """Seeded particle-image pair (frame A, frame B) as numpy arrays of shape (h, w)."""
+    dt = np.dtype(dtype)
+    d = SynthDesc()
+    d.w, d.h, d.seed = w, h, seed
+    d.density, d.diameter = density, diameter
+    d.i0_lo, d.i0_hi = i0
+    d.noise_sigma = noise
+    d.field = field
+    p = list(params) + [0.0] * (4 - len(params))
+    d.p0, d.p1, d.p2, d.p3 = p[:4]
+    d.dtype = _SYNTH_CODES[dt]
+    d.nthreads = threads or min(16, os.cpu_count() or 1)
+    a = np.empty((h, w), dtype=dt)
+    b = np.empty((h, w), dtype=dt)
+    if _load_synth().opiv_synth_pair(ctypes.byref(d), a.ctypes.data, b.ctypes.data) != 0:
+        raise ValueError("opiv_synth_pair rejected the descriptor")
+    return a, b

@@ -1,0 +1,255 @@
+"""GPU parity: the HIP engine, called through the C ABI, against the FP64
+oracle and the reference's golden vectors.
+
+Bar (stated once, used everywhere below):
+  * integer peaks (px, py) equal, except windows the engine flags
+    NEAR_TIE / EXACT_TIE (top-2 or peak-vs-neighbour gap below the FP32
+    tolerance); those are counted and must stay rare;
+  * |u - u_ref|, |v - v_ref| <= 1e-3 px on every other window;
+  * s/n within 1e-4 relative; x, y exact; FEWER_THAN_2_PEAKS identical;
+  * correlation planes within 1e-5 of the plane's max |value|.
+"""
+import numpy as np
+import pytest
+
+import openpiv_amd as oa
+import oracle as orc
+from conftest import load_npz, synth_from_meta
+
+pytestmark = pytest.mark.gpu
+
+PX_TOL = 1e-3
+SN_RTOL = 1e-4
+PLANE_RTOL = 1e-5
+MAX_TIE_FRACTION = 0.01
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def compare(res, ref, label):
+    """res: engine results (RESULT_DTYPE); ref: oracle VECTOR_DTYPE or golden dict."""
+    get = (lambda k: ref[k]) if isinstance(ref, np.ndarray) else (lambda k: ref[f"{label}_{k}"])
+    assert len(res) == len(get("u"))
+    few_ref = (get("flags") & 1) != 0
+    few = (res["flags"] & oa.FLAG_FEWER_THAN_2_PEAKS) != 0
+    assert np.array_equal(few, few_ref), (label, np.flatnonzero(few != few_ref)[:8])
+    tie = (res["flags"] & (oa.FLAG_NEAR_TIE | oa.FLAG_EXACT_TIE)) != 0
+    ok = ~few & ~tie
+    assert tie.mean() <= MAX_TIE_FRACTION, (label, tie.sum())
+    bad_peak = ok & ((res["px"] != get("px")) | (res["py"] != get("py")))
+    assert not bad_peak.any(), (label, np.flatnonzero(bad_peak)[:8], res[bad_peak][:3])
+    fin = ok & np.isfinite(get("u")) & np.isfinite(get("v"))
+    du = np.abs(res["u"][fin] - get("u")[fin])
+    dv = np.abs(res["v"][fin] - get("v")[fin])
+    err = max(du.max(initial=0), dv.max(initial=0))
+    assert err <= PX_TOL, (label, err)
+    # a non-finite reference fit is non-finite (and flagged) here too
+    nonfin = ok & ~(np.isfinite(get("u")) & np.isfinite(get("v")))
+    assert ((res["flags"][nonfin] & oa.FLAG_NONFINITE_FIT) != 0).all()
+    sn_ref = get("sn")[fin]
+    assert np.allclose(res["sn"][fin], sn_ref, rtol=SN_RTOL, atol=0), label
+    assert np.array_equal(res["x"][~few], get("x")[~few]) and np.array_equal(res["y"][~few], get("y")[~few])
+    assert (res["u"][few] == 0).all() and (res["x"][few] == 0).all()
+    rms = float(np.sqrt(np.mean(np.concatenate([du, dv]) ** 2))) if du.size else 0.0
+    say(f"[{label}] windows={len(res)} max|d|={err:.3e}px rms={rms:.3e}px near_ties={int(tie.sum())} "
+        f"few_peaks={int(few.sum())}")
+    return err, int(tie.sum())
+
+
+# ------------------------------------------------------------------ planes
+
+@pytest.mark.parametrize("win", [8, 16, 32, 64])
+@pytest.mark.parametrize("variant", [oa.CIRCULAR_FFT_FP32, oa.REAL_COMPAT])
+def test_correlation_planes(gpu_ctx, win, variant):
+    say(f"planes win={win} variant={variant}")
+    rng = np.random.default_rng(win + 10 * variant)
+    n = 37                                   # not a multiple of windows per block
+    a = rng.random((n, win, win)) * 200
+    b = np.roll(a, (1, -2), (1, 2)) + rng.random((n, win, win)) * 20
+    got = gpu_ctx.cross_correlate(a, b, variant)
+    f = orc.FFT(win)
+    for i in range(n):
+        ref = f.cross_correlate(a[i], b[i]) if variant == 0 else f.cross_correlate_real(a[i], b[i])
+        err = np.abs(got[i] - ref).max() / np.abs(ref).max()
+        assert err <= PLANE_RTOL, (i, err)
+
+
+@pytest.mark.parametrize("win", [16, 32])
+def test_auto_correlation_planes(gpu_ctx, win):
+    say(f"auto win={win}")
+    rng = np.random.default_rng(win)
+    a = rng.random((5, win, win)) * 100
+    got = gpu_ctx.auto_correlate(a)
+    f = orc.FFT(win)
+    for i in range(5):
+        ref = f.auto_correlate(a[i])
+        assert np.abs(got[i] - ref).max() <= PLANE_RTOL * np.abs(ref).max()
+
+
+def test_planes_match_reference_goldens(gpu_ctx):
+    g = load_npz("synth1024_f64.npz")
+    a, b = synth_from_meta("synth")
+    xy = oa.grid(1024, 1024, 32, 32, 16, 16)
+    idx = g["plane_idx"]
+    wa = np.stack([a[xy[i, 1]:xy[i, 1] + 32, xy[i, 0]:xy[i, 0] + 32] for i in idx])
+    wb = np.stack([b[xy[i, 1]:xy[i, 1] + 32, xy[i, 0]:xy[i, 0] + 32] for i in idx])
+    for variant, key in ((0, "planes_c32"), (1, "planes_r32")):
+        got = gpu_ctx.cross_correlate(wa, wb, variant)
+        ref = g[key]
+        assert (np.abs(got - ref).max(axis=(1, 2)) <= PLANE_RTOL * np.abs(ref).max(axis=(1, 2))).all()
+
+
+# ------------------------------------------------------------------ golden vectors
+
+@pytest.mark.parametrize("label,win,step,variant,limit", [
+    ("c32", 32, 16, oa.CIRCULAR_FFT_FP32, False), ("r32", 32, 16, oa.REAL_COMPAT, False),
+    ("l32", 32, 16, oa.CIRCULAR_FFT_FP32, True), ("c64", 64, 32, oa.CIRCULAR_FFT_FP32, False),
+])
+def test_golden_synth1024(gpu_ctx, label, win, step, variant, limit):
+    say(f"golden synth1024 {label}")
+    g = load_npz("synth1024_f64.npz")
+    a, b = synth_from_meta("synth")
+    plan = oa.Plan(gpu_ctx, 1024, 1024, win, step=(step, step), dtype=np.float64, variant=variant,
+                   limit_search=limit)
+    compare(plan.run(a, b), g, label)
+
+
+@pytest.mark.parametrize("label,win,step,variant", [
+    ("c32", 32, 16, oa.CIRCULAR_FFT_FP32), ("r32", 32, 16, oa.REAL_COMPAT),
+    ("c16", 16, 8, oa.CIRCULAR_FFT_FP32), ("c64", 64, 32, oa.CIRCULAR_FFT_FP32),
+])
+def test_golden_reference_example_frames(gpu_ctx, label, win, step, variant):
+    say(f"golden piv crop {label}")
+    g = load_npz("piv_crop512_u8.npz")
+    plan = oa.Plan(gpu_ctx, 512, 512, win, step=(step, step), dtype=np.uint8, variant=variant)
+    compare(plan.run(g["a"], g["b"]), g, label)
+
+
+def test_golden_synth512_small_windows(gpu_ctx):
+    g = load_npz("synth512_f64_w16.npz")
+    a, b = synth_from_meta("synth512")
+    for label, win, step in (("c16", 16, 8), ("c8", 8, 4)):
+        say(f"golden synth512 {label}")
+        plan = oa.Plan(gpu_ctx, 512, 512, win, step=(step, step), dtype=np.float64)
+        compare(plan.run(a, b), g, label)
+
+
+# ------------------------------------------------------------------ oracle at size
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32, np.float64])
+def test_dtypes_against_oracle(gpu_ctx, dtype):
+    say(f"dtype {np.dtype(dtype).name}")
+    i0 = (30.0, 250.0) if dtype == np.uint8 else (1000.0, 60000.0)
+    a, b = oa.synth_pair(768, 512, seed=21, i0=i0, noise=3.0 if dtype == np.uint8 else 50.0,
+                         params=(0.4, 1.1), dtype=dtype)
+    plan = oa.Plan(gpu_ctx, 768, 512, 32, overlap=0.5, dtype=dtype)
+    ref = orc.process_pair(a, b, 32, 16, 16, threads=16)
+    compare(plan.run(a, b), ref, f"dtype-{np.dtype(dtype).name}")
+
+
+def rotation_pair(w, dmax=6.0, dtype=np.float32, seed=2):
+    r = np.hypot(w / 2 - 16, w / 2 - 16)
+    return oa.synth_pair(w, w, seed=seed, field=oa.FIELD_ROTATION, params=(dmax, r), dtype=dtype)
+
+
+@pytest.mark.parametrize("win,step", [(32, 16), (64, 32)])
+def test_rotation_4096_f32(gpu_ctx, win, step):
+    """4096^2 float32, solid-body rotation |d|max = 6 px: every window
+    against the FP64 oracle."""
+    say(f"rotation 4096 f32 win={win}")
+    a, b = rotation_pair(4096)
+    plan = oa.Plan(gpu_ctx, 4096, 4096, win, step=(step, step), dtype=np.float32)
+    res = plan.run(a, b)
+    ref = orc.process_pair(a, b, win, step, step, threads=16)
+    compare(res, ref, f"rot4096-w{win}")
+
+
+@pytest.mark.parametrize("win,step", [(16, 8), (32, 16)])
+def test_uint16_4096(gpu_ctx, win, step):
+    say(f"uint16 4096 win={win}")
+    a, b = oa.synth_pair(4096, 4096, seed=4, i0=(1000.0, 60000.0), noise=50.0, params=(0.4, 1.1),
+                         dtype=np.uint16)
+    plan = oa.Plan(gpu_ctx, 4096, 4096, win, step=(step, step), dtype=np.uint16)
+    ref = orc.process_pair(a, b, win, step, step, threads=16)
+    compare(plan.run(a, b), ref, f"u16-4096-w{win}")
+
+
+def test_limit_search_and_real_against_oracle(gpu_ctx):
+    a, b = oa.synth_pair(1024, 768, seed=8, params=(-3.2, 5.1), dtype=np.float32)
+    for variant, limit in ((oa.REAL_COMPAT, True), (oa.CIRCULAR_FFT_FP32, True), (oa.REAL_COMPAT, False)):
+        say(f"variant={variant} limit={limit}")
+        plan = oa.Plan(gpu_ctx, 1024, 768, 32, dtype=np.float32, variant=variant, limit_search=limit)
+        ref = orc.process_pair(a, b, 32, 16, 16, variant=variant, limit_search=limit, threads=16)
+        compare(plan.run(a, b), ref, f"v{variant}-l{int(limit)}")
+
+
+# ------------------------------------------------------------------ edges
+
+def test_single_window_and_margins(gpu_ctx):
+    for (w, h), win, step in (((32, 32), 32, 16), ((100, 50), 32, 16), ((65, 65), 16, 8), ((70, 33), 8, 4)):
+        say(f"edge {w}x{h} win={win}")
+        a, b = oa.synth_pair(w, h, seed=w + h, params=(1.2, 0.7))
+        plan = oa.Plan(gpu_ctx, w, h, win, step=(step, step), dtype=np.float64)
+        compare(plan.run(a, b), orc.process_pair(a, b, win, step, step, threads=4), f"edge{w}x{h}")
+
+
+def test_pitch_larger_than_width(gpu_ctx):
+    a, b = oa.synth_pair(512, 256, seed=6, params=(2.0, -1.0), dtype=np.float32)
+    wide_a = np.zeros((256, 600), np.float32)
+    wide_b = np.zeros((256, 600), np.float32)
+    wide_a[:, :512] = a
+    wide_b[:, :512] = b
+    plan = oa.Plan(gpu_ctx, 512, 256, 32, dtype=np.float32, pitch=600)
+    res = plan.run(wide_a, wide_b)
+    compare(res, orc.process_pair(a, b, 32, 16, 16, threads=4), "pitch600")
+
+
+def test_blank_constant_and_negative_frames(gpu_ctx):
+    plan = oa.Plan(gpu_ctx, 128, 128, 32, dtype=np.float32)
+    zero = np.zeros((128, 128), np.float32)
+    res = plan.run(zero, zero)
+    assert (res["flags"] & oa.FLAG_FEWER_THAN_2_PEAKS).all() and (res["u"] == 0).all()
+    const = np.full((128, 128), 7.0, np.float32)
+    res_c = plan.run(const, const)
+    ref_c = orc.process_pair(const, const, 32, 16, 16, threads=2)
+    assert np.array_equal((res_c["flags"] & 1) != 0, (ref_c["flags"] & 1) != 0)
+    rng = np.random.default_rng(3)
+    neg = (rng.random((128, 128)) - 0.5).astype(np.float32)
+    res_n = plan.run(neg, np.roll(neg, 1, 1))
+    ref_n = orc.process_pair(neg, np.roll(neg, 1, 1), 32, 16, 16, threads=2)
+    compare(res_n, ref_n, "negative")
+
+
+def test_device_pointers_and_determinism(gpu_ctx):
+    torch = pytest.importorskip("torch")
+    a, b = oa.synth_pair(1024, 1024, seed=12, params=(1.5, 0.5), dtype=np.float32)
+    plan = oa.Plan(gpu_ctx, 1024, 1024, 32, dtype=np.float32)
+    host = plan.run(a, b)
+    da = torch.from_numpy(a).cuda()
+    db = torch.from_numpy(b).cuda()
+    out = torch.zeros(plan.count * oa.RESULT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    plan.run(da, db, out)
+    dev = out.cpu().numpy().view(oa.RESULT_DTYPE)
+    assert host.tobytes() == dev.tobytes()
+    assert plan.run(a, b).tobytes() == host.tobytes()
+
+
+def test_errors(gpu_ctx):
+    with pytest.raises(oa.OpivError, match="dimensions must be power of 2") as e:
+        oa.Plan(gpu_ctx, 256, 256, 24)
+    assert e.value.code == 2
+    with pytest.raises(oa.OpivError) as e:
+        oa.Plan(gpu_ctx, 256, 256, 128)
+    assert e.value.code == 6
+    with pytest.raises(oa.OpivError, match="offsets must be between"):
+        oa.Plan(gpu_ctx, 256, 256, 32, overlap=1.5)
+    with pytest.raises(oa.OpivError, match="bigger than image"):
+        oa.Plan(gpu_ctx, 16, 256, 32)
+    plan = oa.Plan(gpu_ctx, 256, 256, 32)
+    a = np.zeros((256, 256), np.float32)
+    bad = np.zeros(3, oa.RESULT_DTYPE)
+    rc = oa.load().opiv_run_pair(plan.handle, a.ctypes.data, a.ctypes.data, bad.ctypes.data, 3)
+    assert rc == 3 and "differs" in oa.load().opiv_last_error().decode()
