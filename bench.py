@@ -47,7 +47,8 @@ class HipEvents:
     """hipEvent timing on an arbitrary stream through libamdhip64."""
 
     def __init__(self):
-        self.hip = ctypes.CDLL("libamdhip64.so")
+        import openpiv_amd as oa
+        self.hip = oa._hip_runtime()
         self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
         self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
@@ -150,7 +151,6 @@ def main():
               "(multi-GPU sharding is a design note, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import openpiv_amd as oa
 
     win = args.win
@@ -159,34 +159,35 @@ def main():
     ctx = oa.Context(0)
     plan = oa.Plan(ctx, w, w, win, step=(step, step), dtype=np.float32)
     n = plan.count
-    da = torch.from_numpy(a).to("cuda:0")
-    db = torch.from_numpy(b).to("cuda:0")
-    dout = torch.zeros(n * RESULT_BYTES, dtype=torch.uint8, device="cuda:0")
-    torch.cuda.synchronize()
+    # frames resident in HBM before timing (device buffers from the engine's
+    # own HIP runtime)
+    da, db = oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)
+    dout = oa.DeviceBuffer(n * RESULT_BYTES)
 
     ev = HipEvents()
     stream = ctx.stream
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
-        plan.run_async(da.data_ptr(), db.data_ptr(), dout.data_ptr())
+        plan.run_async(da.ptr, db.ptr, dout.ptr)
     ctx.synchronize()
-    torch.cuda.synchronize()
 
     log(f"[bench] timing {args.steps} steps")
     e0, e1 = ev.create(), ev.create()
+    hip = oa._hip_runtime()
+    hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
     ev.record(e0, stream)
     for _ in range(args.steps):
-        plan.run_async(da.data_ptr(), db.data_ptr(), dout.data_ptr())
+        plan.run_async(da.ptr, db.ptr, dout.ptr)
     ev.record(e1, stream)
     ctx.synchronize()
-    torch.cuda.synchronize()
+    hip.hipDeviceSynchronize()
     wall = time.perf_counter() - t0
     ms = ev.elapsed_ms(e0, e1)
     ms_per_step = ms / args.steps
     value = n / (ms_per_step * 1e-3)
 
-    res = dout.cpu().numpy().view(oa.RESULT_DTYPE)
+    res = dout.download(np.zeros(n, oa.RESULT_DTYPE))
     log("[bench] accuracy sample vs FP64 oracle")
     acc = accuracy(res, a, b, win, step)
 

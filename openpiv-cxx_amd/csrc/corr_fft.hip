@@ -212,11 +212,15 @@ __global__ __launch_bounds__(kCorrThreads) void corr_fft_kernel(CorrParams p)
             const float l = from_left(c);
             const float r = from_right(c);
             const bool ismax = col_ok && h >= hlo && h <= hhi && l < c && r < c && s[h - 1] < c && s[h + 1] < c;
-            if (ismax) {
-                const int idx = h * W + t;
-                if (c > v1) { v2 = v1; i2 = i1; v1 = c; i1 = idx; }
-                else if (c > v2) { v2 = c; i2 = idx; }
-            }
+            // branchless top-2 update; rows arrive in scan order, so a later
+            // equal value never displaces an earlier one
+            const int idx = h * W + t;
+            const bool gt1 = ismax && c > v1;
+            const bool gt2 = ismax && !gt1 && c > v2;
+            v2 = gt1 ? v1 : (gt2 ? c : v2);
+            i2 = gt1 ? i1 : (gt2 ? idx : i2);
+            v1 = gt1 ? c : v1;
+            i1 = gt1 ? idx : i1;
         });
         // top-2 over the W lanes of the window
 #pragma unroll

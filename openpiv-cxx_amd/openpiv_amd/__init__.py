@@ -281,6 +281,81 @@ class Plan:
         _check(self._lib.opiv_run_pair_async(self.handle, a_ptr, b_ptr, out_ptr, self.count))
 
 
+# ---------------------------------------------------------------- device memory
+
+_hip = None
+
+
+def _hip_runtime():
+    """The HIP runtime libopenpiv_amd.so is linked against (same soname, so
+    the dynamic loader hands back the already loaded copy).  PyTorch ships a
+    second copy of the runtime; loading both in one process leaves the later
+    one without devices, so device buffers are taken from this one."""
+    global _hip
+    if _hip is None:
+        load()
+        h = ctypes.CDLL("libamdhip64.so.7")
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        h.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
+        h.hipFree.argtypes = [vp]
+        h.hipMemcpy.argtypes = [vp, vp, sz, ctypes.c_int]
+        h.hipMemset.argtypes = [vp, ctypes.c_int, sz]
+        h.hipDeviceSynchronize.argtypes = []
+        for f in (h.hipMalloc, h.hipFree, h.hipMemcpy, h.hipMemset, h.hipDeviceSynchronize):
+            f.restype = ctypes.c_int
+        _hip = h
+    return _hip
+
+
+class DeviceBuffer:
+    """Device allocation (hipMalloc) with host copies; ``ptr`` is the device address."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self, nbytes: int):
+        self._h = _hip_runtime()
+        p = ctypes.c_void_p()
+        rc = self._h.hipMalloc(ctypes.byref(p), max(1, nbytes))
+        if rc != 0:
+            raise OpivError(5, f"hipMalloc({nbytes}) failed: {rc}")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes)
+        buf.upload(a)
+        return buf
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if self._h.hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, self.H2D) != 0:
+            raise OpivError(5, "hipMemcpy H2D failed")
+
+    def download(self, out: np.ndarray) -> np.ndarray:
+        assert out.flags.c_contiguous and out.nbytes <= self.nbytes
+        if self._h.hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, self.D2H) != 0:
+            raise OpivError(5, "hipMemcpy D2H failed")
+        return out
+
+    def zero(self) -> None:
+        if self._h.hipMemset(self.ptr, 0, self.nbytes) != 0:
+            raise OpivError(5, "hipMemset failed")
+
+    def free(self) -> None:
+        if self.ptr:
+            self._h.hipFree(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------- synthetic frames
 
 FIELD_UNIFORM, FIELD_ROTATION, FIELD_LAMB_OSEEN = 0, 1, 2

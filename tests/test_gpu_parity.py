@@ -6,7 +6,8 @@ Bar (stated once, used everywhere below):
     NEAR_TIE / EXACT_TIE (top-2 or peak-vs-neighbour gap below the FP32
     tolerance); those are counted and must stay rare;
   * |u - u_ref|, |v - v_ref| <= 1e-3 px on every other window;
-  * s/n within 1e-4 relative; x, y exact; FEWER_THAN_2_PEAKS identical;
+  * peak heights within 1e-5 of peak0; s/n within 1e-5 * (1 + peak0/|peak1|)
+    relative; x, y exact; FEWER_THAN_2_PEAKS identical;
   * correlation planes within 1e-5 of the plane's max |value|.
 """
 import numpy as np
@@ -19,7 +20,7 @@ from conftest import load_npz, synth_from_meta
 pytestmark = pytest.mark.gpu
 
 PX_TOL = 1e-3
-SN_RTOL = 1e-4
+SN_RTOL = 1e-5
 PLANE_RTOL = 1e-5
 MAX_TIE_FRACTION = 0.01
 
@@ -48,8 +49,16 @@ def compare(res, ref, label):
     # a non-finite reference fit is non-finite (and flagged) here too
     nonfin = ok & ~(np.isfinite(get("u")) & np.isfinite(get("v")))
     assert ((res["flags"][nonfin] & oa.FLAG_NONFINITE_FIT) != 0).all()
+    # FP32 correlation error is absolute, of order PLANE_RTOL * peak0, so the
+    # peak heights are compared against peak0 and s/n = peak0/peak1 gets the
+    # matching relative bound SN_RTOL * (1 + peak0/|peak1|)
+    p0 = get("peak0")[fin]
+    p1 = get("peak1")[fin]
+    assert (np.abs(res["peak0"][fin] - p0) <= PLANE_RTOL * np.abs(p0)).all(), label
+    assert (np.abs(res["peak1"][fin] - p1) <= PLANE_RTOL * np.abs(p0)).all(), label
     sn_ref = get("sn")[fin]
-    assert np.allclose(res["sn"][fin], sn_ref, rtol=SN_RTOL, atol=0), label
+    sn_tol = SN_RTOL * (1.0 + np.abs(p0) / np.maximum(np.abs(p1), 1e-300)) * np.abs(sn_ref)
+    assert (np.abs(res["sn"][fin] - sn_ref) <= sn_tol).all(), label
     assert np.array_equal(res["x"][~few], get("x")[~few]) and np.array_equal(res["y"][~few], get("y")[~few])
     assert (res["u"][few] == 0).all() and (res["x"][few] == 0).all()
     rms = float(np.sqrt(np.mean(np.concatenate([du, dv]) ** 2))) if du.size else 0.0
@@ -223,17 +232,20 @@ def test_blank_constant_and_negative_frames(gpu_ctx):
 
 
 def test_device_pointers_and_determinism(gpu_ctx):
-    torch = pytest.importorskip("torch")
+    """Frames and results in device memory give the same bytes as the host
+    path, and repeated runs are bit-identical."""
     a, b = oa.synth_pair(1024, 1024, seed=12, params=(1.5, 0.5), dtype=np.float32)
     plan = oa.Plan(gpu_ctx, 1024, 1024, 32, dtype=np.float32)
     host = plan.run(a, b)
-    da = torch.from_numpy(a).cuda()
-    db = torch.from_numpy(b).cuda()
-    out = torch.zeros(plan.count * oa.RESULT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
-    torch.cuda.synchronize()
-    plan.run(da, db, out)
-    dev = out.cpu().numpy().view(oa.RESULT_DTYPE)
+    da, db = oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)
+    dout = oa.DeviceBuffer(plan.count * oa.RESULT_DTYPE.itemsize)
+    plan.run(da.ptr, db.ptr, dout.ptr)
+    dev = dout.download(np.zeros(plan.count, oa.RESULT_DTYPE))
     assert host.tobytes() == dev.tobytes()
+    dout.zero()
+    plan.run_async(da.ptr, db.ptr, dout.ptr)
+    gpu_ctx.synchronize()
+    assert dout.download(np.zeros(plan.count, oa.RESULT_DTYPE)).tobytes() == host.tobytes()
     assert plan.run(a, b).tobytes() == host.tobytes()
 
 
