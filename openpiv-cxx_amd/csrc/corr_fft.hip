@@ -11,7 +11,7 @@
 // with the correlation plane kept in registers: it never reaches HBM.
 //
 // Work mapping.  A window of W x W pixels is owned by W consecutive lanes of
-// one wavefront (64 / W windows per wave).  Lane t loads pixel column t of
+// one wavefront (64 / W lane groups per wave).  Lane t loads pixel column t of
 // both frames (coalesced: the W lanes read W consecutive pixels of a row),
 // packs z = a + i b, and runs a W-point FFT down its column in registers.
 // One LDS transpose gives each lane a spectral row for the second FFT; an
@@ -22,6 +22,10 @@
 // that lands in quadrant-swapped column t, so the swapped plane is again a
 // register renaming.  find_peaks then runs on registers with DPP neighbour
 // moves, a cross-lane top-2 reduction follows, and the fit uses FP64 logs.
+// For W <= 32 each lane group runs two windows: the second window's spectrum
+// is added as i * P1 to the first's, so one inverse FFT yields both real
+// planes (real and imaginary parts), and the FP64 logs of both stencils are
+// spread over the group's lanes, one log per lane.
 // LDS traffic stays inside a wave, so only wave-level syncs are needed.
 #include <hip/hip_runtime.h>
 
@@ -68,11 +72,6 @@ __device__ __forceinline__ float load_px(const void* base, size_t idx)
     else return (float)((const double*)base)[idx];
 }
 
-struct Cand {
-    float v;
-    int i;
-};
-
 // ordering of find_peaks' sort (descending value); equal values keep scan
 // order, i.e. the smaller row-major index comes first
 __device__ __forceinline__ bool better(float va, int ia, float vb, int ib)
@@ -80,21 +79,130 @@ __device__ __forceinline__ bool better(float va, int ia, float vb, int ib)
     return va > vb || (va == vb && ia < ib);
 }
 
-__device__ __forceinline__ double gauss3(double l, double c, double r)
-{
-    // image_utils.impl.h:86-94, same operation order
-    const double num = log(l) - log(r);
-    const double den = 2.0 * (log(l) + log(r) - 2.0 * log(c));
-    if (den == 0.0) return 0.0;
-    return num / den;
-}
-
 } // namespace
 
-template <int W, int DT, bool PLANES>
+// Load window (x0, y0) column t of both frames, transform, and leave in q
+// (natural order, lane t = spectral row ky) the cross-power spectrum
+// P = B conj(A) of this lane's row -- or, with ACC, add i 2^k P to q.
+// Returns the window's max |Z| component (Z = FFT(a + i b)), which bounds
+// |P| <= max|Z|^2.  The power of two 2^k equalises that bound between the
+// two packed spectra (zmax0 is the first window's), so each plane's FP32
+// rounding stays relative to its own magnitude; scaling by 2^k is exact and
+// the caller divides it out of the second plane.
+template <int W, int DT, bool ACC>
+__device__ __forceinline__ float forward_product(const CorrParams& p, int x0, int y0, int t,
+                                                 float2 (*lds)[W + 1], float2 (&q)[W],
+                                                 float zmax0, float& scale)
+{
+    constexpr int H2 = W / 2;
+    // ---- load column t: z[y] = a + i b ----
+    float2 z[W];
+    {
+        const size_t idx = (size_t)y0 * p.pitch + (size_t)(x0 + t);
+        sfor<W>([&](auto Y) {
+            constexpr int y = decltype(Y)::value;
+            z[y] = make_float2(load_px<DT>(p.a, idx + (size_t)y * p.pitch),
+                               load_px<DT>(p.b, idx + (size_t)y * p.pitch));
+        });
+    }
+    // ---- FFT along y (column t), transpose through LDS ----
+    fftreg::fft_dif<W, -1>(z);
+    wave_sync();                                  // earlier readers of this buffer are done
+    sfor<W>([&](auto K) {
+        constexpr int ky = decltype(K)::value;
+        lds[ky][t] = z[brev_n<W>(ky)];
+    });
+    wave_sync();
+    sfor<W>([&](auto X) {
+        constexpr int x = decltype(X)::value;
+        z[x] = lds[t][x];
+    });
+    // ---- FFT along x: lane t holds spectral row ky = t; publish it ----
+    fftreg::fft_dif<W, -1>(z);
+    wave_sync();
+    sfor<W>([&](auto K) {
+        constexpr int kx = decltype(K)::value;
+        lds[t][kx] = z[brev_n<W>(kx)];
+    });
+    wave_sync();
+    // max |Z| over the window (lanes of the group)
+    float zmax = 0.f;
+    sfor<W>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        zmax = fmaxf(zmax, fmaxf(fabsf(z[k].x), fabsf(z[k].y)));
+    });
+#pragma unroll
+    for (int off = W / 2; off >= 1; off >>= 1) zmax = fmaxf(zmax, __shfl_xor(zmax, off, 64));
+    scale = 1.f;
+    if constexpr (ACC) {
+        int e0, e1;
+        frexpf(zmax0, &e0);
+        frexpf(zmax, &e1);
+        const int k = (zmax0 == 0.f || zmax == 0.f) ? 0 : max(-60, min(60, 2 * (e0 - e1)));
+        scale = ldexpf(1.f, k);
+    }
+    const float f = 0.25f * scale;
+    // ---- separate the two real spectra and form P = B conj(A) ----
+    const int m = (W - t) & (W - 1);              // row of -ky
+    const bool compat_row = p.real_compat && (t == 0 || t == H2);
+    sfor<W>([&](auto K) {
+        constexpr int kx = decltype(K)::value;
+        constexpr int kxm = (W - kx) & (W - 1);
+        const float2 zk = z[brev_n<W>(kx)];
+        const float2 zr = lds[m][kxm];            // Z(-k)
+        // s = Z + conj Z(-k) = 2A ; d = Z - conj Z(-k) = 2iB ; P = -i d conj(s) / 4
+        const float sr = zk.x + zr.x, si = zk.y - zr.y;
+        const float dr = zk.x - zr.x, di = zk.y + zr.y;
+        float pr = f * (di * sr - dr * si);
+        float pi = -f * (dr * sr + di * si);
+        // fft.h:166-183 leaves ky in {0, H/2} and kx = 0 empty
+        if (compat_row || (kx == 0 && p.real_compat)) { pr = 0.f; pi = 0.f; }
+        if constexpr (ACC) q[kx] = make_float2(q[kx].x - pi, q[kx].y + pr);
+        else q[kx] = make_float2(pr, pi);
+    });
+    return zmax;
+}
+
+// find_peaks(.., 2, 1) over one quadrant-swapped plane column held by lane
+// t: strict 4-neighbour maxima on rows [hlo, hhi] and columns [wlo, whi]
+// (image_utils.impl.h:48-64), reduced to the largest value (and its scan
+// index h*W + t; equal values keep scan order) and the second largest value
+// over the W lanes of the window.
+template <int W>
+__device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, int hhi, bool col_ok,
+                                          float& v1, int& i1, float& v2)
+{
+    v1 = -INFINITY;
+    v2 = -INFINITY;
+    int h1 = 0x7fff;
+    sfor<W - 3>([&](auto Hm) {
+        constexpr int h = decltype(Hm)::value + 1;
+        const float c = s[h];
+        const float nb = fmaxf(fmaxf(from_left(c), from_right(c)), fmaxf(s[h - 1], s[h + 1]));
+        const bool ismax = col_ok && h >= hlo && h <= hhi && c > nb;
+        const float mc = ismax ? c : -INFINITY;
+        const bool gt = mc > v1;                  // rows arrive in scan order: ties keep the first
+        v2 = fmaxf(v2, fminf(mc, v1));
+        h1 = gt ? h : h1;
+        v1 = gt ? mc : v1;
+    });
+    i1 = h1 * W + t;
+#pragma unroll
+    for (int off = W / 2; off >= 1; off >>= 1) {
+        const float ov1 = __shfl_xor(v1, off, 64);
+        const float ov2 = __shfl_xor(v2, off, 64);
+        const int oi1 = __shfl_xor(i1, off, 64);
+        v2 = fmaxf(fmaxf(v2, ov2), fminf(v1, ov1));
+        const bool mine = better(v1, i1, ov1, oi1);
+        v1 = mine ? v1 : ov1;
+        i1 = mine ? i1 : oi1;
+    }
+}
+
+template <int W, int DT, bool PLANES, int ROUNDS>
 __global__ __launch_bounds__(kCorrThreads) void corr_fft_kernel(CorrParams p)
 {
-    constexpr int WPW = 64 / W;                 // windows per wave
+    constexpr int WPW = 64 / W;                 // windows per wave and round
     constexpr int WAVES = kCorrThreads / 64;
     constexpr int LS = W + 1;                   // padded LDS row (float2)
     constexpr int H2 = W / 2;
@@ -106,63 +214,29 @@ __global__ __launch_bounds__(kCorrThreads) void corr_fft_kernel(CorrParams p)
     const int t = lane % W;
     float2 (*lds)[LS] = lds_all[wave][slot];
 
-    const int local = (blockIdx.x * WAVES + wave) * WPW + slot;
-    const bool active = local < p.n_windows;
-    const int gid = p.first + (active ? local : p.n_windows - 1);
-    const int ix = gid % p.nx, iy = gid / p.nx;
-    const int x0 = p.x_start + ix * p.step_x;
-    const int y0 = p.y_start + iy * p.step_y;
-
-    // ---- load column t of both windows: z[y] = a + i b ----
-    float2 z[W];
-    {
-        size_t idx = (size_t)y0 * p.pitch + (size_t)(x0 + t);
-        sfor<W>([&](auto Y) {
-            constexpr int y = decltype(Y)::value;
-            z[y] = make_float2(load_px<DT>(p.a, idx + (size_t)y * p.pitch), load_px<DT>(p.b, idx + (size_t)y * p.pitch));
-        });
+    // round r of this lane group handles window local[r]
+    int local[ROUNDS], x0[ROUNDS], y0[ROUNDS];
+    bool active[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        local[r] = ((blockIdx.x * WAVES + wave) * ROUNDS + r) * WPW + slot;
+        active[r] = local[r] < p.n_windows;
+        const int gid = p.first + (active[r] ? local[r] : p.n_windows - 1);
+        const int ix = gid % p.nx, iy = gid / p.nx;
+        x0[r] = p.x_start + ix * p.step_x;
+        y0[r] = p.y_start + iy * p.step_y;
     }
 
-    // ---- forward FFT along y (column t), then transpose ----
-    fftreg::fft_dif<W, -1>(z);
-    sfor<W>([&](auto K) {
-        constexpr int ky = decltype(K)::value;
-        lds[ky][t] = z[brev_n<W>(ky)];
-    });
-    wave_sync();
-    sfor<W>([&](auto X) {
-        constexpr int x = decltype(X)::value;
-        z[x] = lds[t][x];
-    });
-
-    // ---- forward FFT along x: lane t holds spectral row ky = t ----
-    fftreg::fft_dif<W, -1>(z);
-    wave_sync();
-    sfor<W>([&](auto K) {
-        constexpr int kx = decltype(K)::value;
-        lds[t][kx] = z[brev_n<W>(kx)];
-    });
-    wave_sync();
-
-    // ---- separate the two real spectra and form P = B conj(A) ----
-    const int m = (W - t) & (W - 1);            // row of -ky
-    const bool compat_row = p.real_compat && (t == 0 || t == H2);
+    // ---- forward transforms; with two rounds the spectra pack as
+    // Q = P0 + i P1, whose inverse is c0 + i c1 (both planes are real) ----
+    const int gbase = lane - t;
     float2 q[W];
-    sfor<W>([&](auto K) {
-        constexpr int kx = decltype(K)::value;
-        constexpr int kxm = (W - kx) & (W - 1);
-        const float2 zk = z[brev_n<W>(kx)];
-        const float2 zr = lds[m][kxm];           // Z(-k)
-        // s = Z + conj Z(-k) = 2A ; d = Z - conj Z(-k) = 2iB
-        const float sr = zk.x + zr.x, si = zk.y - zr.y;
-        const float dr = zk.x - zr.x, di = zk.y + zr.y;
-        // P = -i d conj(s) / 4
-        float pr = 0.25f * (di * sr - dr * si);
-        float pi = -0.25f * (dr * sr + di * si);
-        // fft.h:166-183 leaves ky in {0, H/2} and kx = 0 empty
-        if (compat_row || (kx == 0 && p.real_compat)) { pr = 0.f; pi = 0.f; }
-        q[kx] = make_float2(pr, pi);
-    });
+    float scale = 1.f, inv_scale = 1.f;
+    const float zmax0 = forward_product<W, DT, false>(p, x0[0], y0[0], t, lds, q, 0.f, scale);
+    if constexpr (ROUNDS == 2) {
+        forward_product<W, DT, true>(p, x0[1], y0[1], t, lds, q, zmax0, scale);
+        inv_scale = 1.f / scale;
+    }
 
     // ---- inverse along x (row ky = t), transpose, inverse along y ----
     fftreg::fft_dif<W, +1>(q);
@@ -179,127 +253,135 @@ __global__ __launch_bounds__(kCorrThreads) void corr_fft_kernel(CorrParams p)
     });
     fftreg::fft_dif<W, +1>(q);
 
-    // swapped plane, column t: S[h] = c[(h + W/2) mod W]
-    float s[W];
+    // quadrant-swapped planes, column t: S[h] = c[(h + W/2) mod W]
+    float s[ROUNDS][W];
     sfor<W>([&](auto Hh) {
         constexpr int h = decltype(Hh)::value;
-        s[h] = q[brev_n<W>((h + H2) & (W - 1))].x;
+        const float2 v = q[brev_n<W>((h + H2) & (W - 1))];
+        s[0][h] = v.x;
+        if constexpr (ROUNDS == 2) s[ROUNDS - 1][h] = v.y * inv_scale;
     });
 
     if constexpr (PLANES) {
-        if (active) {
-            double* dst = p.planes + (size_t)local * W * W + t;
-            sfor<W>([&](auto Hh) {
-                constexpr int h = decltype(Hh)::value;
-                dst[h * W] = (double)s[h];
-            });
-        }
+        sfor<ROUNDS>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            if (active[r]) {
+                double* dst = p.planes + (size_t)local[r] * W * W + t;
+                sfor<W>([&](auto Hh) {
+                    constexpr int h = decltype(Hh)::value;
+                    dst[h * W] = (double)s[r][h];
+                });
+            }
+        });
         return;
     } else {
-        // ---- find_peaks(.., 2, 1): strict 4-neighbour maxima over rows
-        // [ry+1, ry+rh-2) and columns [rx+1, rx+rw-1) of the swapped plane
-        // (image_utils.impl.h:48-64; the limit-search view is rect::dilate(0.5)) ----
+        // search region: whole plane, or the limit-search view rect::dilate(0.5)
         const int r0 = p.limit_search ? W / 4 : 0;
         const int rlen = p.limit_search ? W / 2 : W;
-        const int hlo = r0 + 1, hhi = r0 + rlen - 3;   // inclusive
-        const int wlo = r0 + 1, whi = r0 + rlen - 2;
-        const bool col_ok = t >= wlo && t <= whi;
-        float v1 = -INFINITY, v2 = -INFINITY;
-        int i1 = 0x7fffffff, i2 = 0x7fffffff;
-        sfor<W - 3>([&](auto Hm) {
-            constexpr int h = decltype(Hm)::value + 1;
-            const float c = s[h];
-            const float l = from_left(c);
-            const float r = from_right(c);
-            const bool ismax = col_ok && h >= hlo && h <= hhi && l < c && r < c && s[h - 1] < c && s[h + 1] < c;
-            // branchless top-2 update; rows arrive in scan order, so a later
-            // equal value never displaces an earlier one
-            const int idx = h * W + t;
-            const bool gt1 = ismax && c > v1;
-            const bool gt2 = ismax && !gt1 && c > v2;
-            v2 = gt1 ? v1 : (gt2 ? c : v2);
-            i2 = gt1 ? i1 : (gt2 ? idx : i2);
-            v1 = gt1 ? c : v1;
-            i1 = gt1 ? idx : i1;
-        });
-        // top-2 over the W lanes of the window
-#pragma unroll
-        for (int off = W / 2; off >= 1; off >>= 1) {
-            const float ov1 = __shfl_xor(v1, off, 64), ov2 = __shfl_xor(v2, off, 64);
-            const int oi1 = __shfl_xor(i1, off, 64), oi2 = __shfl_xor(i2, off, 64);
-            if (better(v1, i1, ov1, oi1)) {
-                if (!better(v2, i2, ov1, oi1)) { v2 = ov1; i2 = oi1; }
-            } else {
-                const bool keep_mine = better(v1, i1, ov2, oi2);
-                v2 = keep_mine ? v1 : ov2;
-                i2 = keep_mine ? i1 : oi2;
-                v1 = ov1;
-                i1 = oi1;
-            }
-        }
-        const bool two = i2 != 0x7fffffff;
-        const int px = two ? (i1 & (W - 1)) : 1;
-        const int py = two ? (i1 / W) : 1;
-        // stencil values around the peak: centre column from lane px,
-        // horizontal neighbours from lanes px -+ 1
-        float mid = 0.f, up = 0.f, dn = 0.f;
-        sfor<W - 2>([&](auto Hm) {
-            constexpr int h = decltype(Hm)::value + 1;
-            if (h == py) { mid = s[h]; up = s[h - 1]; dn = s[h + 1]; }
-        });
-        const int gbase = lane - t;
-        const float fl = __shfl(mid, gbase + px - 1, 64);
-        const float fc = __shfl(mid, gbase + px, 64);
-        const float fr = __shfl(mid, gbase + px + 1, 64);
-        const float fu = __shfl(up, gbase + px, 64);
-        const float fd = __shfl(dn, gbase + px, 64);
+        const int hlo = r0 + 1, hhi = r0 + rlen - 3;
+        const bool col_ok = t >= r0 + 1 && t <= r0 + rlen - 2;
 
-        if (active && t == 0) {
+        float v1[ROUNDS], v2[ROUNDS], st[ROUNDS][5];
+        int px[ROUNDS], py[ROUNDS];
+        bool two[ROUNDS];
+        sfor<ROUNDS>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            int i1;
+            scan_top2<W>(s[r], t, hlo, hhi, col_ok, v1[r], i1, v2[r]);
+            two[r] = v2[r] > -INFINITY;
+            px[r] = two[r] ? (i1 & (W - 1)) : 1;
+            py[r] = two[r] ? (i1 / W) : 1;
+            // stencil (l, c, r, up, down) around the peak: the centre column
+            // comes from lane px, the horizontal neighbours from lanes px -+ 1
+            float mid = 0.f, up = 0.f, dn = 0.f;
+            sfor<W - 2>([&](auto Hm) {
+                constexpr int h = decltype(Hm)::value + 1;
+                const bool at = h == py[r];
+                mid = at ? s[r][h] : mid;
+                up = at ? s[r][h - 1] : up;
+                dn = at ? s[r][h + 1] : dn;
+            });
+            st[r][0] = __shfl(mid, gbase + px[r] - 1, 64);
+            st[r][1] = __shfl(mid, gbase + px[r], 64);
+            st[r][2] = __shfl(mid, gbase + px[r] + 1, 64);
+            st[r][3] = __shfl(up, gbase + px[r], 64);
+            st[r][4] = __shfl(dn, gbase + px[r], 64);
+        });
+
+        // ---- FP64 logs of the 5 * ROUNDS stencil values, one per lane ----
+        constexpr int NV = 5 * ROUNDS;
+        constexpr int NP = (NV + W - 1) / W;
+        double lg[NP];
+        sfor<NP>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            float v = 1.f;
+            sfor<NV>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                v = (t + k * W == j) ? st[j / 5][j % 5] : v;
+            });
+            lg[k] = log((double)v);
+        });
+        double ln[NV];
+        sfor<NV>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            ln[j] = __shfl(lg[j / W], gbase + (j % W), 64);
+        });
+
+        sfor<ROUNDS>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            if (!(active[r] && t == r)) return;
             opiv_window_result res;
             res.reserved = 0;
-            if (!two) {
+            if (!two[r]) {
                 // main.cpp:214-218: the default (all-zero) record stays
                 res.x = res.y = res.u = res.v = res.sn = 0.0;
                 res.peak0 = res.peak1 = 0.0;
                 res.px = res.py = 0;
                 res.flags = OPIV_FLAG_FEWER_THAN_2_PEAKS;
             } else {
+                // image_utils.impl.h:86-98 on the logs of (l, c, r) and (up, c, down)
+                const double lnl = ln[5 * r + 0], lnc = ln[5 * r + 1], lnr = ln[5 * r + 2];
+                const double lnu = ln[5 * r + 3], lnd = ln[5 * r + 4];
+                const double numx = lnl - lnr, denx = 2.0 * (lnl + lnr - 2.0 * lnc);
+                const double numy = lnu - lnd, deny = 2.0 * (lnu + lnd - 2.0 * lnc);
+                const double plx = (double)px[r] + (denx == 0.0 ? 0.0 : numx / denx);
+                const double ply = (double)py[r] + (deny == 0.0 ? 0.0 : numy / deny);
                 uint32_t flags = 0;
-                // image_utils.impl.h:96-98: x from (l, c, r), y from rows py-1, py, py+1
-                const double plx = (double)px + gauss3((double)fl, (double)fc, (double)fr);
-                const double ply = (double)py + gauss3((double)fu, (double)fc, (double)fd);
                 if (!isfinite(plx) || !isfinite(ply)) flags |= OPIV_FLAG_NONFINITE_FIT;
-                const float nmax = fmaxf(fmaxf(fl, fr), fmaxf(fu, fd));
-                const float tol = p.tie_tol * fabsf(v1);
-                if (v1 - v2 <= tol || v1 - nmax <= tol) flags |= OPIV_FLAG_NEAR_TIE;
-                if (v1 == v2) flags |= OPIV_FLAG_EXACT_TIE;
+                const float nmax = fmaxf(fmaxf(st[r][0], st[r][2]), fmaxf(st[r][3], st[r][4]));
+                const float tol = p.tie_tol * fabsf(v1[r]);
+                if (v1[r] - v2[r] <= tol || v1[r] - nmax <= tol) flags |= OPIV_FLAG_NEAR_TIE;
+                if (v1[r] == v2[r]) flags |= OPIV_FLAG_EXACT_TIE;
                 // main.cpp:220-234
-                const int midx = x0 + W / 2, midy = y0 + W / 2;
+                const int midx = x0[r] + W / 2, midy = y0[r] + W / 2;
                 res.x = (double)midx;
                 res.y = (double)p.img_h - (double)midy;
-                res.u = (double)midx - ((double)x0 + plx);
-                res.v = (double)midy - ((double)y0 + ply);
-                res.sn = (v2 > 0.f) ? (double)v1 / (double)v2 : 0.0;
-                res.peak0 = (double)v1;
-                res.peak1 = (double)v2;
-                res.px = px;
-                res.py = py;
+                res.u = (double)midx - ((double)x0[r] + plx);
+                res.v = (double)midy - ((double)y0[r] + ply);
+                res.sn = (v2[r] > 0.f) ? (double)v1[r] / (double)v2[r] : 0.0;
+                res.peak0 = (double)v1[r];
+                res.peak1 = (double)v2[r];
+                res.px = px[r];
+                res.py = py[r];
                 res.flags = flags;
             }
-            p.out[local] = res;
-        }
+            p.out[local[r]] = res;
+        });
     }
 }
 
 template <int W, int DT>
 static hipError_t launch_w(const CorrParams& p, bool planes, hipStream_t stream)
 {
-    constexpr int per_block = (kCorrThreads / 64) * (64 / W);
+    // two windows per lane group (packed inverse) while the second set of
+    // spectra fits in registers
+    constexpr int ROUNDS = W <= 32 ? 2 : 1;
+    constexpr int per_block = (kCorrThreads / 64) * ROUNDS * (64 / W);
     const int blocks = (p.n_windows + per_block - 1) / per_block;
     if (planes)
-        hipLaunchKernelGGL((corr_fft_kernel<W, DT, true>), dim3(blocks), dim3(kCorrThreads), 0, stream, p);
+        hipLaunchKernelGGL((corr_fft_kernel<W, DT, true, ROUNDS>), dim3(blocks), dim3(kCorrThreads), 0, stream, p);
     else
-        hipLaunchKernelGGL((corr_fft_kernel<W, DT, false>), dim3(blocks), dim3(kCorrThreads), 0, stream, p);
+        hipLaunchKernelGGL((corr_fft_kernel<W, DT, false, ROUNDS>), dim3(blocks), dim3(kCorrThreads), 0, stream, p);
     return hipGetLastError();
 }
 
