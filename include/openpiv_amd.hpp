@@ -1,0 +1,175 @@
+// openpiv_amd.hpp -- header-only C++17 layer over the C ABI of openpiv_amd.h.
+//
+// Mirrors the interface examples/process uses from openpiv-cxx so the engine
+// drops into that driver:
+//   * openpiv_amd::HipFFT          ~ algos::FFT (algos/fft.h:28-295): the
+//     constructor checks power-of-two sizes with the same message
+//     (fft.h:82-83); cross_correlate / cross_correlate_real / auto_correlate
+//     take and return images of the caller's type (fft.h:188-242), so the
+//     correlator_t registry of main.cpp:153-186 can hold it unchanged.
+//   * openpiv_amd::Plan            ~ the grid (main.cpp:137-138) plus the whole
+//     per-window loop (main.cpp:189-302) as one batched call per frame pair.
+// Errors surface as std::runtime_error subclasses carrying the C status, as
+// the reference throws std::runtime_error (exception_builder.h:12-31).
+//
+// ImageT requirements (satisfied by openpiv::core::image<g<double>>):
+// width(), height(), operator[](size_t) readable as double and assignable
+// from double, and construction as ImageT(width, height).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "openpiv_amd.h"
+
+namespace openpiv_amd {
+
+class error : public std::runtime_error {
+public:
+    error(int code, const std::string& what) : std::runtime_error(what), code_(code) {}
+    int code() const noexcept { return code_; }
+
+private:
+    int code_;
+};
+
+inline void check(int rc)
+{
+    if (rc != OPIV_OK) throw error(rc, opiv_last_error());
+}
+
+/// One device and one HIP stream (opiv_ctx).
+class Context {
+public:
+    explicit Context(int device = 0) { check(opiv_ctx_create(device, &h_)); }
+    ~Context() { opiv_ctx_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    opiv_ctx* get() const { return h_; }
+    void synchronize() const { check(opiv_ctx_synchronize(h_)); }
+    /// serialises host threads that share this context
+    std::mutex& mutex() const { return m_; }
+
+private:
+    opiv_ctx* h_ = nullptr;
+    mutable std::mutex m_;
+};
+
+/// Square-window FFT correlator with the call surface of algos::FFT.
+/// Thread-safe like the reference class: calls from several threads are
+/// serialised on the context.
+class HipFFT {
+public:
+    HipFFT(Context& ctx, uint32_t width, uint32_t height) : ctx_(ctx), w_(width), h_(height)
+    {
+        const auto pow2 = [](uint32_t v) { return v && (v & (v - 1)) == 0; };
+        if (!(pow2(w_) && pow2(h_)))
+            throw error(OPIV_E_NOT_POW2, "dimensions must be power of 2: [" + std::to_string(w_) + "," +
+                                             std::to_string(h_) + "]");
+        if (w_ != h_)
+            throw error(OPIV_E_UNSUPPORTED, "only square windows are supported: [" + std::to_string(w_) + "," +
+                                                std::to_string(h_) + "]");
+    }
+
+    /// fft.h:188-206: real(IFFT(FFT(b) conj(FFT(a)))), quadrant-swapped
+    template <class ImageT>
+    ImageT cross_correlate(const ImageT& a, const ImageT& b) const
+    {
+        return correlate(a, &b, OPIV_CIRCULAR_FFT_FP32);
+    }
+
+    /// fft.h:208-227 (with the reference's two-real-image unpack)
+    template <class ImageT>
+    ImageT cross_correlate_real(const ImageT& a, const ImageT& b) const
+    {
+        return correlate(a, &b, OPIV_REAL_COMPAT);
+    }
+
+    /// fft.h:229-242 (real part)
+    template <class ImageT>
+    ImageT auto_correlate(const ImageT& a) const
+    {
+        return correlate(a, static_cast<const ImageT*>(nullptr), OPIV_CIRCULAR_FFT_FP32);
+    }
+
+    /// Batched form: n stacked w*h windows (host or device memory).
+    void cross_correlate(const double* a, const double* b, size_t n, double* planes,
+                         opiv_variant variant = OPIV_CIRCULAR_FFT_FP32) const
+    {
+        std::lock_guard<std::mutex> lock(ctx_.mutex());
+        check(opiv_correlate_windows(ctx_.get(), w_, n, variant, a, b, planes));
+    }
+
+private:
+    template <class ImageT>
+    ImageT correlate(const ImageT& a, const ImageT* b, opiv_variant variant) const
+    {
+        if (a.width() != w_ || a.height() != h_ || (b && (b->width() != w_ || b->height() != h_)))
+            throw error(OPIV_E_SIZE, "image size is different from expected: [" + std::to_string(a.width()) + "," +
+                                         std::to_string(a.height()) + "], [" + std::to_string(w_) + "," +
+                                         std::to_string(h_) + "]");
+        const size_t n = static_cast<size_t>(w_) * h_;
+        std::vector<double> da(n), db(b ? n : 0), out(n);
+        for (size_t i = 0; i < n; ++i) da[i] = static_cast<double>(a[i]);
+        if (b)
+            for (size_t i = 0; i < n; ++i) db[i] = static_cast<double>((*b)[i]);
+        {
+            std::lock_guard<std::mutex> lock(ctx_.mutex());
+            if (b)
+                check(opiv_correlate_windows(ctx_.get(), w_, 1, variant, da.data(), db.data(), out.data()));
+            else
+                check(opiv_auto_correlate_windows(ctx_.get(), w_, 1, da.data(), out.data()));
+        }
+        ImageT result(w_, h_);
+        for (size_t i = 0; i < n; ++i) result[i] = out[i];
+        return result;
+    }
+
+    Context& ctx_;
+    uint32_t w_, h_;
+};
+
+/// Frame size + window + grid; run() is the batched processing loop.
+class Plan {
+public:
+    Plan(Context& ctx, const opiv_plan_desc& desc) : ctx_(ctx) { check(opiv_plan_create(ctx.get(), &desc, &h_)); }
+    ~Plan() { opiv_plan_destroy(h_); }
+    Plan(const Plan&) = delete;
+    Plan& operator=(const Plan&) = delete;
+
+    size_t window_count() const
+    {
+        size_t n = 0;
+        check(opiv_plan_window_count(h_, &n));
+        return n;
+    }
+
+    /// bottom-left corners in grid order (generate_cartesian_grid)
+    std::vector<int32_t> grid() const
+    {
+        std::vector<int32_t> xy(2 * window_count());
+        check(opiv_plan_grid(h_, xy.data()));
+        return xy;
+    }
+
+    /// one frame pair (host or device pointers) -> one result per window
+    std::vector<opiv_window_result> run(const void* a, const void* b) const
+    {
+        std::vector<opiv_window_result> out(window_count());
+        std::lock_guard<std::mutex> lock(ctx_.mutex());
+        check(opiv_run_pair(h_, a, b, out.data(), out.size()));
+        return out;
+    }
+
+    opiv_plan* get() const { return h_; }
+
+private:
+    Context& ctx_;
+    opiv_plan* h_ = nullptr;
+};
+
+} // namespace openpiv_amd

@@ -1,0 +1,249 @@
+// process_main.cpp -- drop-in front-end of examples/process (main.cpp:34-317)
+// over the HIP engine.
+//
+// Same options and defaults (main.cpp:53-63): -s/--size 32, -o/--overlap 0.5,
+// -i/--input (or two positional files), -t/--thread-count, -e/--exec
+// (pool | bulk-pool | async++ | single), -l/--limit-search,
+// -f/--ffttype (complex | real | pocket | pocket_real), --loglevel.  Thread
+// count and execution strategy select host-thread schedules in the reference;
+// here every window of the pair goes through one GPU launch, so they are
+// accepted and ignored.  --device picks the GPU.
+//
+// Frames are read with a PNM reader following the reference's loader
+// (loaders/pnm_image_loader.cpp:279-310; P5, 8- or 16-bit big-endian, '#'
+// comments before each header number) and are handed to the device as
+// 8/16-bit pixels, without the FP64 widening of the reference.
+//
+// Output matches main.cpp:312-313: one "x, y, u, v, sn" line per window in
+// grid order, at the stream's default precision.
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "openpiv_amd.hpp"
+
+namespace {
+
+enum class Level { NONE, ERROR, WARN, INFO, DEBUG };
+
+Level g_level = Level::INFO;
+
+void log(Level l, const std::string& m)
+{
+    if (l <= g_level && l != Level::NONE) std::cerr << m << "\n";
+}
+
+struct Frame {
+    uint32_t width = 0, height = 0;
+    bool wide = false;              // 16-bit samples
+    std::vector<uint8_t> px8;
+    std::vector<uint16_t> px16;
+    const void* data() const { return wide ? (const void*)px16.data() : (const void*)px8.data(); }
+};
+
+std::istream& skip_comments(std::istream& is)
+{
+    // pnm_image_loader.cpp:39-48
+    while (is.peek() == '#') is.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
+    return is;
+}
+
+bool load_pnm(const std::string& path, Frame& f, std::string& err)
+{
+    std::ifstream is(path, std::ios::binary);
+    if (!is.is_open()) {
+        err = "failed to open " + path;
+        return false;
+    }
+    char magic[1024] = {};
+    is.getline(magic, sizeof(magic));
+    if (!(magic[0] == 'P' && (magic[1] == '5' || magic[1] == '6'))) {
+        err = "failed to find loader for " + path;
+        return false;
+    }
+    if (magic[1] == '6') {
+        err = "failed to load image: " + path + " is colour (P6); only greyscale P5 frames are supported";
+        return false;
+    }
+    uint32_t depth = 0;
+    skip_comments(is) >> f.width;
+    skip_comments(is) >> f.height;
+    skip_comments(is) >> depth;
+    is.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
+    if (!is || f.width == 0 || f.height == 0) {
+        err = "failed to load image: bad PNM header in " + path;
+        return false;
+    }
+    const size_t n = (size_t)f.width * f.height;
+    f.wide = depth > 255;
+    if (f.wide) {
+        std::vector<uint8_t> raw(2 * n);
+        is.read(reinterpret_cast<char*>(raw.data()), (std::streamsize)raw.size());
+        f.px16.resize(n);
+        for (size_t i = 0; i < n; ++i) f.px16[i] = (uint16_t)((raw[2 * i] << 8) | raw[2 * i + 1]);   // big-endian
+    } else {
+        f.px8.resize(n);
+        is.read(reinterpret_cast<char*>(f.px8.data()), (std::streamsize)n);
+    }
+    if (!is) {
+        err = "failed to load image: short pixel data in " + path;
+        return false;
+    }
+    return true;
+}
+
+void widen(Frame& f)
+{
+    if (f.wide) return;
+    f.px16.assign(f.px8.begin(), f.px8.end());
+    f.px8.clear();
+    f.wide = true;
+}
+
+bool take_value(int& i, int argc, char** argv, const std::string& arg, const std::string& shortf,
+                const std::string& longf, std::string& value)
+{
+    if (arg == shortf || arg == longf) {
+        if (i + 1 >= argc) return false;
+        value = argv[++i];
+        return true;
+    }
+    if (arg.rfind(longf + "=", 0) == 0) {
+        value = arg.substr(longf.size() + 1);
+        return true;
+    }
+    if (shortf.size() == 2 && arg.size() > 2 && arg.rfind(shortf, 0) == 0 && arg[2] != '-') {
+        value = arg.substr(2);
+        return true;
+    }
+    return false;
+}
+
+} // namespace
+
+int main(int argc, char* argv[])
+{
+    uint32_t size = 32;
+    double overlap = 0.5;
+    std::vector<std::string> inputs;
+    std::string execution = "pool", fft_type = "complex", level = "INFO";
+    bool limit_search = false;
+    int device = 0;
+
+    for (int i = 1; i < argc; ++i) {
+        const std::string arg = argv[i];
+        std::string v;
+        try {
+            if (arg == "-h" || arg == "--help") {
+                std::cout << "usage: " << argv[0]
+                          << " [-s size] [-o overlap] [-t threads] [-e exec] [-l] [-f complex|real|pocket|pocket_real]"
+                             " [--loglevel LEVEL] [--device N] -i A B | A B\n";
+                return 0;
+            } else if (take_value(i, argc, argv, arg, "-s", "--size", v)) {
+                size = (uint32_t)std::stoul(v);
+            } else if (take_value(i, argc, argv, arg, "-o", "--overlap", v)) {
+                overlap = std::stod(v);
+            } else if (take_value(i, argc, argv, arg, "-t", "--thread-count", v)) {
+                (void)std::stoul(v);
+            } else if (take_value(i, argc, argv, arg, "-e", "--exec", v)) {
+                execution = v;
+            } else if (take_value(i, argc, argv, arg, "-f", "--ffttype", v)) {
+                fft_type = v;
+            } else if (take_value(i, argc, argv, arg, "", "--loglevel", v)) {
+                level = v;
+            } else if (take_value(i, argc, argv, arg, "", "--device", v)) {
+                device = std::stoi(v);
+            } else if (arg == "-l" || arg == "--limit-search") {
+                limit_search = true;
+            } else if (take_value(i, argc, argv, arg, "-i", "--input", v)) {
+                inputs.push_back(v);
+            } else if (!arg.empty() && arg[0] == '-') {
+                std::cerr << "error parsing options: unknown option " << arg << "\n";
+                return 1;
+            } else {
+                inputs.push_back(arg);
+            }
+        } catch (const std::exception& e) {
+            std::cerr << "error parsing options: " << e.what() << "\n";
+            return 1;
+        }
+    }
+    if (level == "NONE") g_level = Level::NONE;
+    else if (level == "ERROR") g_level = Level::ERROR;
+    else if (level == "WARN") g_level = Level::WARN;
+    else if (level == "DEBUG") g_level = Level::DEBUG;
+
+    if (inputs.size() != 2) {
+        log(Level::ERROR, "require two input images");
+        return 1;
+    }
+    log(Level::INFO, "size: " + std::to_string(size) + " x " + std::to_string(size));
+    log(Level::INFO, "overlap: " + std::to_string(overlap));
+    log(Level::INFO, "input files: " + inputs[0] + ", " + inputs[1]);
+    log(Level::INFO, "execution: " + execution + " (one GPU launch per frame pair)");
+
+    Frame fa, fb;
+    std::string err;
+    if (!load_pnm(inputs[0], fa, err) || !load_pnm(inputs[1], fb, err)) {
+        log(Level::ERROR, err);
+        return 1;
+    }
+    if (fa.width != fb.width || fa.height != fb.height) {
+        log(Level::ERROR, "image sizes don't match: [" + std::to_string(fa.width) + "," + std::to_string(fa.height) +
+                              "], [" + std::to_string(fb.width) + "," + std::to_string(fb.height) + "]");
+        return 1;
+    }
+    if (fa.wide != fb.wide) {
+        widen(fa);
+        widen(fb);
+    }
+
+    int variant;
+    if (fft_type == "complex" || fft_type == "pocket" || fft_type == "pocket_real") {
+        variant = OPIV_CIRCULAR_FFT_FP32;   // the same exact circular correlation
+    } else if (fft_type == "real") {
+        variant = OPIV_REAL_COMPAT;
+    } else {
+        log(Level::ERROR, "unknown fft type: " + fft_type);
+        return 1;
+    }
+
+    try {
+        openpiv_amd::Context ctx(device);
+        opiv_plan_desc desc{};
+        desc.img_w = fa.width;
+        desc.img_h = fa.height;
+        desc.pitch = 0;
+        desc.dtype = fa.wide ? OPIV_U16 : OPIV_U8;
+        desc.win = size;
+        desc.overlap = overlap;
+        desc.variant = variant;
+        desc.limit_search = limit_search ? 1 : 0;
+        openpiv_amd::Plan plan(ctx, desc);
+        log(Level::INFO, "grid count: " + std::to_string(plan.window_count()));
+
+        const auto t1 = std::chrono::high_resolution_clock::now();
+        const auto results = plan.run(fa.data(), fb.data());
+        const auto t2 = std::chrono::high_resolution_clock::now();
+        const std::chrono::duration<double, std::micro> us = t2 - t1;
+        log(Level::INFO, "processing time: " + std::to_string(us.count()) + "us, " +
+                             std::to_string(us.count() / (double)results.size()) + "us per interrogation area");
+        size_t failed = 0;
+        for (const auto& r : results)
+            if (r.flags & OPIV_FLAG_FEWER_THAN_2_PEAKS) ++failed;
+        if (failed) log(Level::ERROR, "failed to find a peak for " + std::to_string(failed) + " interrogation areas");
+
+        // main.cpp:312-313
+        for (const auto& r : results) std::cout << r.x << ", " << r.y << ", " << r.u << ", " << r.v << ", " << r.sn << "\n";
+    } catch (const openpiv_amd::error& e) {
+        log(Level::ERROR, e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -1,0 +1,41 @@
+"""C++ drop-in layer without a GPU: the header compiles against an image type
+with the reference's accessors, and the front-end's argument and loader
+errors behave like examples/process (main.cpp:85-132)."""
+import subprocess
+
+import numpy as np
+import pytest
+
+from dropin_util import PROCESS_BIN, build_dropin_check, write_pgm
+
+
+def run(args):
+    return subprocess.run([str(a) for a in args], capture_output=True, text=True, timeout=60)
+
+
+def test_header_compiles_and_reports_missing_device(tmp_path):
+    exe = build_dropin_check(tmp_path)
+    out = run([exe, "nodevice"]).stdout
+    assert out.strip() == "ok" or out.startswith("error 7 ")
+
+
+def test_process_requires_two_inputs(tmp_path):
+    r = run([PROCESS_BIN, "-s", "32", tmp_path / "a.pgm"])
+    assert r.returncode == 1 and "require two input images" in r.stderr
+
+
+def test_process_missing_file(tmp_path):
+    r = run([PROCESS_BIN, tmp_path / "a.pgm", tmp_path / "b.pgm"])
+    assert r.returncode == 1 and "failed to open" in r.stderr
+
+
+def test_process_rejects_size_mismatch_and_bad_type(tmp_path):
+    write_pgm(tmp_path / "a.pgm", np.zeros((64, 64), np.uint8))
+    write_pgm(tmp_path / "b.pgm", np.zeros((64, 32), np.uint8), comment=True)
+    r = run([PROCESS_BIN, tmp_path / "a.pgm", tmp_path / "b.pgm"])
+    assert r.returncode == 1 and "image sizes don't match" in r.stderr
+    write_pgm(tmp_path / "b.pgm", np.zeros((64, 64), np.uint8), comment=True)
+    r = run([PROCESS_BIN, "-f", "fftw", tmp_path / "a.pgm", tmp_path / "b.pgm"])
+    assert r.returncode == 1 and "unknown fft type" in r.stderr
+    r = run([PROCESS_BIN, "--bogus", tmp_path / "a.pgm", tmp_path / "b.pgm"])
+    assert r.returncode == 1 and "error parsing options" in r.stderr

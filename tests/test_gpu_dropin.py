@@ -1,0 +1,86 @@
+"""GPU tests of the C++ drop-in layer: HipFFT (algos::FFT's call surface)
+against the FP64 oracle, and the process front-end against the reference's
+golden vectors and the oracle.  Same bar as test_gpu_parity: planes within
+1e-5 of their max, displacements within 1e-3 px (plus the 6-significant-digit
+print rounding of main.cpp:312-313)."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import openpiv_amd as oa
+import oracle as orc
+from conftest import load_npz
+from dropin_util import PROCESS_BIN, build_dropin_check, parse_process_output, write_pgm
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def dropin(tmp_path_factory):
+    return build_dropin_check(tmp_path_factory.mktemp("dropin"))
+
+
+@pytest.mark.parametrize("variant", ["complex", "real", "auto"])
+def test_hipfft_matches_oracle(dropin, tmp_path, variant):
+    say(f"HipFFT {variant}")
+    rng = np.random.default_rng(4)
+    a, b = rng.random((32, 32)) * 100, rng.random((32, 32)) * 100
+    a.tofile(tmp_path / "a.bin")
+    b.tofile(tmp_path / "b.bin")
+    subprocess.run([str(dropin), "correlate", "32", variant, str(tmp_path / "a.bin"), str(tmp_path / "b.bin"),
+                    str(tmp_path / "o.bin")], check=True, timeout=60)
+    got = np.fromfile(tmp_path / "o.bin", dtype=np.float64).reshape(32, 32)
+    f = orc.FFT(32)
+    ref = {"complex": lambda: f.cross_correlate(a, b), "real": lambda: f.cross_correlate_real(a, b),
+           "auto": lambda: f.auto_correlate(a)}[variant]()
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+def test_hipfft_pow2_message(dropin):
+    out = subprocess.run([str(dropin), "pow2", "24", "32"], capture_output=True, text=True, timeout=60).stdout
+    assert "dimensions must be power of 2: [24,32]" in out
+
+
+def check_output(text, ref_xy_u_v_sn, few):
+    got = parse_process_output(text)
+    assert got.shape == (len(few), 5)
+    x, y, u, v, sn = ref_xy_u_v_sn
+    ok = ~few
+    assert np.array_equal(got[ok, 0], x[ok]) and np.array_equal(got[ok, 1], y[ok])
+    tol = 1e-3 + 1e-5 * np.abs(u[ok])
+    assert (np.abs(got[ok, 2] - u[ok]) <= tol).all()
+    assert (np.abs(got[ok, 3] - v[ok]) <= 1e-3 + 1e-5 * np.abs(v[ok])).all()
+    assert np.allclose(got[ok, 4], sn[ok], rtol=1e-4, atol=1e-5)
+    assert (got[few] == 0).all()
+
+
+@pytest.mark.parametrize("label,size,ftype", [("c32", 32, "complex"), ("r32", 32, "real"),
+                                              ("c16", 16, "pocket"), ("c64", 64, "pocket_real")])
+def test_process_on_reference_example_frames(tmp_path, label, size, ftype):
+    say(f"process {label} -f {ftype}")
+    g = load_npz("piv_crop512_u8.npz")
+    write_pgm(tmp_path / "a.pgm", g["a"])
+    write_pgm(tmp_path / "b.pgm", g["b"], comment=True)
+    r = subprocess.run([str(PROCESS_BIN), "-s", str(size), "-f", ftype, str(tmp_path / "a.pgm"),
+                        str(tmp_path / "b.pgm")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    few = (g[f"{label}_flags"] & 1) != 0
+    check_output(r.stdout, [g[f"{label}_{k}"] for k in ("x", "y", "u", "v", "sn")], few)
+
+
+def test_process_16bit_limit_search_and_overlap(tmp_path):
+    a, b = oa.synth_pair(640, 480, seed=31, i0=(1000.0, 60000.0), noise=50.0, params=(1.3, -2.2), dtype=np.uint16)
+    write_pgm(tmp_path / "a.pgm", a)
+    write_pgm(tmp_path / "b.pgm", b)
+    for args, limit, off in ((["-o", "0.75"], False, 24), (["-l"], True, 16), (["--limit-search", "-s", "32"], True, 16)):
+        say(f"process 16-bit {args}")
+        r = subprocess.run([str(PROCESS_BIN)] + args + [str(tmp_path / "a.pgm"), str(tmp_path / "b.pgm")],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        ref = orc.process_pair(a, b, 32, off, off, limit_search=limit, threads=8)
+        check_output(r.stdout, [ref[k] for k in ("x", "y", "u", "v", "sn")], (ref["flags"] & 1) != 0)
