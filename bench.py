@@ -127,6 +127,21 @@ def accuracy(res, a, b, win, step, sample=4096, seed=0):
             "near_ties": int(tie.sum())}
 
 
+def copy_calibration(mib: int, reps: int):
+    """Device-to-device copies of a known size: the runtime's copy kernel
+    reads and writes exactly mib MiB per launch, which scales the FETCH_SIZE /
+    WRITE_SIZE counters to bytes."""
+    import openpiv_amd as oa
+    n = mib << 20
+    src, dst = oa.DeviceBuffer(n), oa.DeviceBuffer(n)
+    src.zero()
+    hip = oa._hip_runtime()
+    for _ in range(reps):
+        assert hip.hipMemcpy(dst.ptr, src.ptr, n, 3) == 0      # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "reps": reps}), flush=True)
+
+
 def load_traffic(key: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, copy-kernel calibrated), committed under profiles/."""
@@ -145,6 +160,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--win", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -152,6 +169,10 @@ def main():
         sys.exit(2)
 
     import openpiv_amd as oa
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, max(1, args.steps))
+        return
 
     win = args.win
     log(f"[bench] generating 4096x4096 f32 pair, win={win}")
