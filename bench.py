@@ -142,6 +142,60 @@ def copy_calibration(mib: int, reps: int):
     print(json.dumps({"copy_calibration_mib": mib, "reps": reps}), flush=True)
 
 
+BATCH_SIZES = [(512, 512), (1024, 1024), (2048, 2048), (4096, 4096), (1280, 800)]
+
+
+def bench_batch(steps: int, warmup: int):
+    """24 frame pairs with seeded sizes from BATCH_SIZES, each with a seeded
+    Lamb-Oseen vortex (core radius 40 px, peak 5 px), passes 64 -> 32 -> 16 at
+    50% overlap, all windows in one graph launch per step."""
+    import openpiv_amd as oa
+    rng = np.random.default_rng(24)
+    picks = rng.integers(0, len(BATCH_SIZES), 24)
+    bufs, pairs = [], []
+    log("[bench] generating 24 pairs:", [BATCH_SIZES[k] for k in picks])
+    for i, k in enumerate(picks):
+        w, h = BATCH_SIZES[k]
+        cx, cy = rng.uniform(0.25, 0.75) * w, rng.uniform(0.25, 0.75) * h
+        a, b = oa.synth_pair(w, h, seed=1000 + i, field=oa.FIELD_LAMB_OSEEN, params=(cx, cy, 40.0, 5.0),
+                             dtype=np.float32)
+        da, db = oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)
+        bufs.append((da, db))
+        pairs.append((da.ptr, db.ptr, w, h))
+    ctx = oa.Context(0)
+    t0 = time.perf_counter()
+    batch = oa.Batch(ctx, pairs, (64, 32, 16))
+    tune_s = time.perf_counter() - t0
+    dout = oa.DeviceBuffer(batch.count * RESULT_BYTES)
+    for _ in range(warmup):
+        batch.run(dout.ptr)
+    ctx.synchronize()
+    ev = HipEvents()
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, ctx.stream)
+    for _ in range(steps):
+        batch.run(dout.ptr)
+    ev.record(e1, ctx.stream)
+    ctx.synchronize()
+    ms = ev.elapsed_ms(e0, e1) / steps
+    npx = sum(w * h for (_, _, w, h) in pairs)
+    per_pass = [int(batch.offsets[(p + 1) * len(pairs)] - batch.offsets[p * len(pairs)]) for p in range(3)]
+    line = {"metric": "vectors/sec, 24-pair multipass batch (64/32/16, 50% overlap), one graph launch",
+            "value": round(batch.count / (ms * 1e-3), 1), "unit": "vectors/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "24 f32 pairs, sizes seeded from 512^2/1024^2/2048^2/4096^2/1280x800, "
+                                   "Lamb-Oseen vortex r_c 40 px peak 5 px, passes 64/32/16 at 50%",
+                       "pairs": len(pairs), "pixels_per_frame_total": npx, "windows": batch.count,
+                       "windows_per_pass": per_pass,
+                       "shapes": [batch.shape(p) for p in range(3)], "create_and_tune_s": round(tune_s, 2)},
+            "roofline": {"bound": "hbm", "achieved": round((3 * 2 * 4 * npx + batch.count * RESULT_BYTES)
+                                                           / (ms * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "note": "algorithmic bytes: every pass reads both frames once"}}
+    print(json.dumps(line), flush=True)
+
+
 def load_traffic(key: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, copy-kernel calibrated), committed under profiles/."""
@@ -160,6 +214,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--win", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--batch", action="store_true",
+                    help="24 uneven frame pairs x passes 64/32/16 in one graph launch (prints its own line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -172,6 +228,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, max(1, args.steps))
+        return
+    if args.batch:
+        bench_batch(args.steps, args.warmup)
         return
 
     win = args.win

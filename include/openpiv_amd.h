@@ -143,6 +143,55 @@ int opiv_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t varian
 int opiv_auto_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n,
                                 const double* a, double* planes);
 
+/* ---- batches: many frame pairs and several window passes per launch ----
+ *
+ * The reference processes one pair and one window size per run of
+ * examples/process (main.cpp:137-302) and has no multipass scheme
+ * (README.md:280: independent passes).  A batch holds npairs pairs of
+ * uneven sizes and up to 4 independent passes (e.g. 64, 32, 16); each pass
+ * is one persistent kernel launch over the windows of every pair, and the
+ * passes are captured once into a HIP graph that opiv_batch_run replays. */
+
+typedef struct opiv_pair_desc {
+    const void* a;           /* frame A, device memory */
+    const void* b;           /* frame B, device memory */
+    uint32_t img_w, img_h;
+    size_t pitch;            /* elements per row (0 = img_w) */
+} opiv_pair_desc;
+
+typedef struct opiv_batch_desc {
+    int32_t dtype;           /* opiv_dtype shared by all pairs */
+    uint32_t npasses;        /* 1..4 */
+    uint32_t win[4];         /* square window per pass, power of 2 in [8, 64] */
+    double overlap[4];       /* process -o per pass (grid.impl.h:10-22) */
+    int32_t variant;         /* opiv_variant */
+    int32_t limit_search;
+} opiv_batch_desc;
+
+typedef struct opiv_batch opiv_batch;
+
+/* Builds every (pass, pair) grid, uploads the pair table, and picks each
+ * pass's launch shape (threads per block, persistent blocks) from the tuning
+ * cache, or by timing the candidates on these frames when the cache has no
+ * entry.  The shape never changes a window's arithmetic, so results do not
+ * depend on it.  The cache is the JSON file $OPIV_TUNE_CACHE, else
+ * $HOME/.cache/openpiv_amd/tune.json; OPIV_TUNE=0 disables timing. */
+int opiv_batch_create(opiv_ctx* ctx, const opiv_batch_desc* desc, const opiv_pair_desc* pairs,
+                      size_t npairs, opiv_batch** out);
+int opiv_batch_destroy(opiv_batch* batch);
+/* total windows over all passes and pairs */
+int opiv_batch_window_count(const opiv_batch* batch, size_t* count);
+/* npasses * npairs + 1 prefix offsets: results of (pass p, pair i) start at
+ * offsets[p * npairs + i], in that pair's grid order */
+int opiv_batch_offsets(const opiv_batch* batch, size_t* offsets);
+/* new frame pointers for the same sizes (takes effect at the next run) */
+int opiv_batch_set_pairs(opiv_batch* batch, const opiv_pair_desc* pairs, size_t npairs);
+/* the chosen launch shape of pass p: threads per block and block count */
+int opiv_batch_shape(const opiv_batch* batch, uint32_t pass, int32_t* threads, int32_t* blocks);
+/* Enqueue every pass as one graph launch on the context stream; d_out is
+ * device memory for opiv_batch_window_count results.  Does not synchronise. */
+int opiv_batch_run(opiv_batch* batch, opiv_window_result* d_out, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,355 @@
+// batch.cpp -- many frame pairs and several window passes per launch.
+//
+// Each pass (one window size over every pair) is a single persistent launch
+// of the fused kernel reading its pairs from a device table; the passes of a
+// batch are captured once into a HIP graph and replayed by opiv_batch_run.
+// Launch shapes (threads per block, persistent block count) come from a
+// tuning cache or are timed on the batch's own frames.  The shape never
+// changes a window's arithmetic: which windows share a packed inverse FFT is
+// fixed by the window numbering, not by the grid, so results are
+// bit-identical for every shape.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <vector>
+
+#include "corr_fft.hpp"
+#include "ctx.hpp"
+#include "openpiv_amd.h"
+
+using namespace opiv::detail;
+
+namespace {
+
+struct Shape {
+    int nt = 256;       // threads per block
+    int blocks = 0;     // 0 = one block per task group; else persistent block count
+};
+
+std::string cache_path()
+{
+    if (const char* p = std::getenv("OPIV_TUNE_CACHE")) return p;
+    const char* home = std::getenv("HOME");
+    return std::string(home ? home : "/tmp") + "/.cache/openpiv_amd/tune.txt";
+}
+
+// cache lines: "<arch> <win> <dtype> <log2 windows> <threads> <blocks per CU x 4>"
+std::string cache_key(const std::string& arch, int win, int dtype, size_t nwin)
+{
+    std::ostringstream k;
+    k << arch << " " << win << " " << dtype << " " << (int)std::lround(std::log2((double)std::max<size_t>(1, nwin)));
+    return k.str();
+}
+
+bool cache_lookup(const std::string& key, int cu, Shape& s)
+{
+    std::ifstream is(cache_path());
+    std::string line;
+    while (std::getline(is, line)) {
+        if (line.rfind(key + " ", 0) != 0) continue;
+        std::istringstream r(line.substr(key.size() + 1));
+        int nt = 0, bpc4 = 0;
+        if (r >> nt >> bpc4 && (nt == 64 || nt == 256) && bpc4 >= 0) {
+            s.nt = nt;
+            s.blocks = bpc4 * cu / 4;
+            return true;
+        }
+    }
+    return false;
+}
+
+void cache_store(const std::string& key, int cu, const Shape& s)
+{
+    const std::string path = cache_path();
+    std::vector<std::string> keep;
+    {
+        std::ifstream is(path);
+        std::string line;
+        while (std::getline(is, line))
+            if (!line.empty() && line.rfind(key + " ", 0) != 0) keep.push_back(line);
+    }
+    for (size_t pos = path.find('/', 1); pos != std::string::npos; pos = path.find('/', pos + 1))
+        mkdir(path.substr(0, pos).c_str(), 0755);
+    const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+    std::ofstream os(tmp);
+    for (const auto& l : keep) os << l << "\n";
+    os << key << " " << s.nt << " " << (cu ? s.blocks * 4 / cu : 0) << "\n";
+    os.close();
+    if (os) std::rename(tmp.c_str(), path.c_str());
+    else std::remove(tmp.c_str());
+}
+
+} // namespace
+
+struct opiv_batch {
+    opiv_ctx* ctx = nullptr;
+    opiv_batch_desc desc{};
+    size_t npairs = 0;
+    std::vector<opiv::BatchPair> table;   // [pass][pair]
+    std::vector<size_t> offsets;          // npasses * npairs + 1
+    std::vector<int32_t> pass_windows;
+    std::vector<Shape> shape;
+    DevBuf d_table;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    opiv_window_result* graph_out = nullptr;
+};
+
+namespace {
+
+// parameters of pass `pass`, writing its results to pass_out
+opiv::CorrParams pass_params(const opiv_batch* b, uint32_t pass, opiv_window_result* pass_out)
+{
+    opiv::CorrParams p{};
+    p.n_windows = b->pass_windows[pass];
+    p.limit_search = b->desc.limit_search ? 1 : 0;
+    p.real_compat = b->desc.variant == OPIV_REAL_COMPAT ? 1 : 0;
+    p.tie_tol = kTieTol;
+    p.out = pass_out;
+    p.pairs = reinterpret_cast<const opiv::BatchPair*>(b->d_table.p) + (size_t)pass * b->npairs;
+    p.npairs = (int32_t)b->npairs;
+    return p;
+}
+
+int fill_table(opiv_batch* b, const opiv_pair_desc* pairs, size_t npairs)
+{
+    const auto& d = b->desc;
+    b->table.assign((size_t)d.npasses * npairs, opiv::BatchPair{});
+    b->offsets.assign((size_t)d.npasses * npairs + 1, 0);
+    b->pass_windows.assign(d.npasses, 0);
+    size_t total = 0;
+    for (uint32_t ps = 0; ps < d.npasses; ++ps) {
+        const uint32_t win = d.win[ps];
+        uint32_t ox, oy;
+        int rc = opiv_grid_offsets(win, win, d.overlap[ps], &ox, &oy);
+        if (rc) return rc;
+        int64_t first = 0;
+        const int64_t align = opiv::batch_align((int)win);
+        const size_t pass_base = total;
+        for (size_t i = 0; i < npairs; ++i) {
+            const auto& pr = pairs[i];
+            if (!pr.a || !pr.b) return fail(OPIV_E_INVALID, "pair %zu: null frame pointer", i);
+            const size_t pitch = pr.pitch ? pr.pitch : pr.img_w;
+            if (pitch < pr.img_w) return fail(OPIV_E_INVALID, "pair %zu: pitch smaller than width", i);
+            uint32_t nx, ny;
+            rc = opiv_grid(pr.img_w, pr.img_h, win, win, ox, oy, &nx, &ny, nullptr);
+            if (rc) return fail(rc, "pair %zu, pass %u: %s", i, ps, opiv_last_error());
+            auto& e = b->table[(size_t)ps * npairs + i];
+            e.a = pr.a;
+            e.b = pr.b;
+            e.pitch = pitch;
+            e.x_start = (int32_t)((pr.img_w - win - ox * (nx - 1)) / 2);
+            e.y_start = (int32_t)((pr.img_h - win - oy * (ny - 1)) / 2);
+            e.step_x = (int32_t)ox;
+            e.step_y = (int32_t)oy;
+            e.nx = (int32_t)nx;
+            e.ny = (int32_t)ny;
+            e.img_h = pr.img_h;
+            e.first = (int32_t)first;
+            e.out_first = (int32_t)(total - pass_base);
+            b->offsets[(size_t)ps * npairs + i] = total;
+            first += ((int64_t)nx * ny + align - 1) / align * align;   // task-aligned range
+            total += (size_t)nx * ny;
+            if (first > 0x7fffffff) return fail(OPIV_E_UNSUPPORTED, "pass %u: too many windows", ps);
+        }
+        b->pass_windows[ps] = (int32_t)first;
+    }
+    b->offsets.back() = total;
+    return OPIV_OK;
+}
+
+int upload_table(opiv_batch* b)
+{
+    const size_t bytes = b->table.size() * sizeof(opiv::BatchPair);
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));   // no graph in flight reads the old table
+    HIP_TRY(b->d_table.ensure(bytes));
+    HIP_TRY(hipMemcpy(b->d_table.p, b->table.data(), bytes, hipMemcpyHostToDevice));
+    return OPIV_OK;
+}
+
+// time each candidate shape on the batch's own frames and keep the fastest
+int tune(opiv_batch* b, uint32_t pass, opiv_window_result* scratch)
+{
+    const int cu = b->ctx->cu_count > 0 ? b->ctx->cu_count : 256;
+    const std::vector<Shape> cands = {{256, 0}, {64, 0}, {256, 2 * cu}, {256, 4 * cu}, {64, 8 * cu}, {64, 16 * cu}};
+    const opiv::CorrParams p = pass_params(b, pass, scratch);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    Shape pick = cands[0];
+    for (const auto& c : cands) {
+        HIP_TRY(opiv::launch_corr_fft_shape(p, (int)b->desc.win[pass], b->desc.dtype, false, c.nt, c.blocks,
+                                            b->ctx->stream));
+        HIP_TRY(hipEventRecord(e0, b->ctx->stream));
+        for (int r = 0; r < 3; ++r)
+            HIP_TRY(opiv::launch_corr_fft_shape(p, (int)b->desc.win[pass], b->desc.dtype, false, c.nt, c.blocks,
+                                                b->ctx->stream));
+        HIP_TRY(hipEventRecord(e1, b->ctx->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->shape[pass] = pick;
+    return OPIV_OK;
+}
+
+void drop_graph(opiv_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_out = nullptr;
+}
+
+} // namespace
+
+extern "C" {
+
+int opiv_batch_create(opiv_ctx* ctx, const opiv_batch_desc* desc, const opiv_pair_desc* pairs, size_t npairs,
+                      opiv_batch** out)
+{
+    if (!ctx || !desc || !pairs || !out || npairs == 0) return fail(OPIV_E_INVALID, "opiv_batch_create: bad argument");
+    *out = nullptr;
+    if (desc->npasses < 1 || desc->npasses > 4) return fail(OPIV_E_INVALID, "npasses must be 1..4");
+    if (dtype_size(desc->dtype) == 0) return fail(OPIV_E_INVALID, "unknown dtype %d", desc->dtype);
+    if (desc->variant != OPIV_CIRCULAR_FFT_FP32 && desc->variant != OPIV_REAL_COMPAT)
+        return fail(OPIV_E_UNSUPPORTED, "unknown variant %d", desc->variant);
+    for (uint32_t ps = 0; ps < desc->npasses; ++ps) {
+        const uint32_t w = desc->win[ps];
+        if (!is_pow2(w)) return fail(OPIV_E_NOT_POW2, "dimensions must be power of 2: [%u,%u]", w, w);
+        if (w < 8 || w > 64) return fail(OPIV_E_UNSUPPORTED, "window %u outside the supported range [8, 64]", w);
+    }
+    if (npairs > 0x7fffffffull) return fail(OPIV_E_UNSUPPORTED, "too many pairs");
+    HIP_TRY(hipSetDevice(ctx->device));
+    opiv_batch* b = new opiv_batch;
+    b->ctx = ctx;
+    b->desc = *desc;
+    b->npairs = npairs;
+    int rc = fill_table(b, pairs, npairs);
+    if (rc == OPIV_OK) rc = upload_table(b);
+    if (rc != OPIV_OK) {
+        opiv_batch_destroy(b);
+        return rc;
+    }
+    b->shape.assign(desc->npasses, Shape{});
+    const char* tune_env = std::getenv("OPIV_TUNE");
+    const bool may_time = !(tune_env && tune_env[0] == '0');
+    DevBuf scratch;
+    for (uint32_t ps = 0; ps < desc->npasses; ++ps) {
+        const size_t pass_results = b->offsets[(size_t)(ps + 1) * npairs] - b->offsets[(size_t)ps * npairs];
+        const std::string key = cache_key(ctx->arch, (int)desc->win[ps], desc->dtype, pass_results);
+        if (cache_lookup(key, ctx->cu_count, b->shape[ps]) || !may_time) continue;
+        hipError_t e = scratch.ensure((size_t)b->pass_windows[ps] * sizeof(opiv_window_result));
+        if (e != hipSuccess) rc = hip_fail(e, "tuning scratch");
+        if (rc == OPIV_OK) rc = tune(b, ps, reinterpret_cast<opiv_window_result*>(scratch.p));
+        if (rc != OPIV_OK) break;
+        cache_store(key, ctx->cu_count, b->shape[ps]);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    scratch.release();
+    if (rc != OPIV_OK) {
+        opiv_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return OPIV_OK;
+}
+
+int opiv_batch_destroy(opiv_batch* batch)
+{
+    if (!batch) return OPIV_OK;
+    (void)hipSetDevice(batch->ctx->device);
+    (void)hipStreamSynchronize(batch->ctx->stream);
+    drop_graph(batch);
+    batch->d_table.release();
+    delete batch;
+    return OPIV_OK;
+}
+
+int opiv_batch_window_count(const opiv_batch* batch, size_t* count)
+{
+    if (!batch || !count) return fail(OPIV_E_INVALID, "opiv_batch_window_count: null argument");
+    *count = batch->offsets.back();
+    return OPIV_OK;
+}
+
+int opiv_batch_offsets(const opiv_batch* batch, size_t* offsets)
+{
+    if (!batch || !offsets) return fail(OPIV_E_INVALID, "opiv_batch_offsets: null argument");
+    std::copy(batch->offsets.begin(), batch->offsets.end(), offsets);
+    return OPIV_OK;
+}
+
+int opiv_batch_set_pairs(opiv_batch* batch, const opiv_pair_desc* pairs, size_t npairs)
+{
+    if (!batch || !pairs) return fail(OPIV_E_INVALID, "opiv_batch_set_pairs: null argument");
+    if (npairs != batch->npairs) return fail(OPIV_E_SIZE, "pair count %zu differs from the batch's %zu", npairs, batch->npairs);
+    std::vector<opiv::BatchPair> old = batch->table;
+    std::vector<size_t> old_offsets = batch->offsets;
+    int rc = fill_table(batch, pairs, npairs);
+    if (rc == OPIV_OK && batch->offsets != old_offsets) rc = fail(OPIV_E_SIZE, "frame sizes differ from the batch's");
+    if (rc != OPIV_OK) {
+        batch->table = old;
+        batch->offsets = old_offsets;
+        return rc;
+    }
+    return upload_table(batch);
+}
+
+int opiv_batch_shape(const opiv_batch* batch, uint32_t pass, int32_t* threads, int32_t* blocks)
+{
+    if (!batch || !threads || !blocks || pass >= batch->desc.npasses)
+        return fail(OPIV_E_INVALID, "opiv_batch_shape: bad argument");
+    *threads = batch->shape[pass].nt;
+    *blocks = batch->shape[pass].blocks;
+    return OPIV_OK;
+}
+
+int opiv_batch_run(opiv_batch* batch, opiv_window_result* d_out, size_t n)
+{
+    if (!batch || !d_out) return fail(OPIV_E_INVALID, "opiv_batch_run: null argument");
+    if (n != batch->offsets.back())
+        return fail(OPIV_E_SIZE, "result count %zu differs from the batch's %zu windows", n, batch->offsets.back());
+    opiv_ctx* ctx = batch->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (!batch->exec || batch->graph_out != d_out) {
+        drop_graph(batch);
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = hipSuccess;
+        for (uint32_t ps = 0; ps < batch->desc.npasses && le == hipSuccess; ++ps)
+            le = opiv::launch_corr_fft_shape(pass_params(batch, ps, d_out + batch->offsets[(size_t)ps * batch->npairs]),
+                                             (int)batch->desc.win[ps],
+                                             batch->desc.dtype, false, batch->shape[ps].nt,
+                                             batch->shape[ps].blocks, ctx->stream);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (le != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hip_fail(le, "batch kernel launch during capture");
+        }
+        HIP_TRY(ce);
+        batch->graph = g;
+        HIP_TRY(hipGraphInstantiate(&batch->exec, batch->graph, nullptr, nullptr, 0));
+        batch->graph_out = d_out;
+    }
+    HIP_TRY(hipGraphLaunch(batch->exec, ctx->stream));
+    return OPIV_OK;
+}
+
+} // extern "C"

@@ -7,106 +7,14 @@
 // reference's wording.
 #include <hip/hip_runtime.h>
 
-#include <cstdarg>
-#include <cstdio>
 #include <cstring>
 #include <string>
 
 #include "corr_fft.hpp"
+#include "ctx.hpp"
 #include "openpiv_amd.h"
 
-namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int code, const char* fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_last_error = buf;
-    return code;
-}
-
-int hip_fail(hipError_t e, const char* what)
-{
-    return fail(OPIV_E_HIP, "%s: %s", what, hipGetErrorString(e));
-}
-
-#define HIP_TRY(call)                                        \
-    do {                                                     \
-        hipError_t e_ = (call);                              \
-        if (e_ != hipSuccess) return hip_fail(e_, #call);    \
-    } while (0)
-
-bool is_pow2(uint64_t v) { return v && (v == (v & (~v + 1))); }   // util.h:67-70
-
-size_t dtype_size(int dt)
-{
-    switch (dt) {
-    case OPIV_U8: return 1;
-    case OPIV_U16: return 2;
-    case OPIV_F32: return 4;
-    case OPIV_F64: return 8;
-    default: return 0;
-    }
-}
-
-// default relative tolerance under which two FP32 correlation values are
-// treated as tied for OPIV_FLAG_NEAR_TIE
-constexpr float kTieTol = 2e-6f;
-
-// device buffer that grows on demand
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes)
-    {
-        if (bytes <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e == hipSuccess) cap = bytes;
-        return e;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-bool is_device_ptr(const void* ptr)
-{
-    hipPointerAttribute_t attr;
-    std::memset(&attr, 0, sizeof(attr));
-    hipError_t e = hipPointerGetAttributes(&attr, ptr);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();   // unregistered host memory: clear the sticky error
-        return false;
-    }
-    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
-}
-
-} // namespace
-
-struct opiv_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    DevBuf stage_a, stage_b, stage_out;
-};
-
-struct opiv_plan {
-    opiv_ctx* ctx = nullptr;
-    opiv_plan_desc desc{};
-    uint32_t off_x = 0, off_y = 0;
-    uint32_t nx = 0, ny = 0;
-    int32_t x_start = 0, y_start = 0;
-};
+using namespace opiv::detail;
 
 extern "C" {
 
@@ -132,6 +40,9 @@ int opiv_ctx_create(int device, opiv_ctx** out)
     HIP_TRY(hipSetDevice(device));
     opiv_ctx* c = new opiv_ctx;
     c->device = device;
+    c->cu_count = prop.multiProcessorCount;
+    c->arch = prop.gcnArchName;
+    c->arch = c->arch.substr(0, c->arch.find(':'));   // "gfx950:sramecc+:xnack-" -> "gfx950"
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;

@@ -63,13 +63,18 @@ __device__ __forceinline__ float from_left(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
 
+// frames are global memory: address-space-1 pointers give global (not flat)
+// loads even when the pointer comes from the batch table
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+
 template <int DT>
 __device__ __forceinline__ float load_px(const void* base, size_t idx)
 {
-    if constexpr (DT == 0) return (float)((const uint8_t*)base)[idx];
-    else if constexpr (DT == 1) return (float)((const uint16_t*)base)[idx];
-    else if constexpr (DT == 2) return ((const float*)base)[idx];
-    else return (float)((const double*)base)[idx];
+    if constexpr (DT == 0) return (float)((gptr<uint8_t>)base)[idx];
+    else if constexpr (DT == 1) return (float)((gptr<uint16_t>)base)[idx];
+    else if constexpr (DT == 2) return ((gptr<float>)base)[idx];
+    else return (float)((gptr<double>)base)[idx];
 }
 
 // ordering of find_peaks' sort (descending value); equal values keep scan
@@ -81,6 +86,69 @@ __device__ __forceinline__ bool better(float va, int ia, float vb, int ib)
 
 } // namespace
 
+// Where one window lives: frames, pitch, bottom-left corner, frame height,
+// the index of its result and whether it is a real window of the launch
+// (task slots past the end of a pair's range recompute that pair's last
+// window and store nothing).
+struct WinGeom {
+    const void* a;
+    const void* b;
+    size_t pitch;
+    int x0, y0;
+    uint32_t img_h;
+    int out;
+    bool valid;
+};
+
+template <class T>
+using gptr_t = const __attribute__((address_space(1))) T*;
+
+// Geometry of window `local` of a launch: the single pair of the params, or
+// (batch) the pair whose window range [first, first + nx * ny) holds it.
+template <bool BATCH>
+__device__ __forceinline__ WinGeom window_geom(const CorrParams& p, int local)
+{
+    WinGeom g;
+    g.valid = local < p.n_windows;
+    if constexpr (!BATCH) {
+        const int l = g.valid ? local : p.n_windows - 1;
+        const int gid = p.first + l;
+        const int ix = gid % p.nx, iy = gid / p.nx;
+        g.a = p.a;
+        g.b = p.b;
+        g.pitch = p.pitch;
+        g.x0 = p.x_start + ix * p.step_x;
+        g.y0 = p.y_start + iy * p.step_y;
+        g.img_h = p.img_h;
+        g.out = l;
+    } else {
+        const gptr_t<BatchPair> pairs = (gptr_t<BatchPair>)p.pairs;
+        const int l = g.valid ? local : p.n_windows - 1;
+        int lo = 0, hi = p.npairs;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (pairs[mid].first <= l) lo = mid;
+            else hi = mid;
+        }
+        const auto& bp = pairs[lo];
+        const int n = bp.nx * bp.ny;
+        int gid = l - bp.first;
+        if (gid >= n) {
+            g.valid = false;
+            gid = n - 1;
+        }
+        const int ix = gid % bp.nx, iy = gid / bp.nx;
+        g.a = bp.a;
+        g.b = bp.b;
+        g.pitch = bp.pitch;
+        g.x0 = bp.x_start + ix * bp.step_x;
+        g.y0 = bp.y_start + iy * bp.step_y;
+        g.img_h = bp.img_h;
+        g.out = bp.out_first + gid;
+    }
+    return g;
+}
+
 // Load window (x0, y0) column t of both frames, transform, and leave in q
 // (natural order, lane t = spectral row ky) the cross-power spectrum
 // P = B conj(A) of this lane's row -- or, with ACC, add i 2^k P to q.
@@ -90,7 +158,7 @@ __device__ __forceinline__ bool better(float va, int ia, float vb, int ib)
 // rounding stays relative to its own magnitude; scaling by 2^k is exact and
 // the caller divides it out of the second plane.
 template <int W, int DT, bool ACC>
-__device__ __forceinline__ float forward_product(const CorrParams& p, int x0, int y0, int t,
+__device__ __forceinline__ float forward_product(const WinGeom& g, bool real_compat, int t,
                                                  float2 (*lds)[W + 1], float2 (&q)[W],
                                                  float zmax0, float& scale)
 {
@@ -98,11 +166,11 @@ __device__ __forceinline__ float forward_product(const CorrParams& p, int x0, in
     // ---- load column t: z[y] = a + i b ----
     float2 z[W];
     {
-        const size_t idx = (size_t)y0 * p.pitch + (size_t)(x0 + t);
+        const size_t idx = (size_t)g.y0 * g.pitch + (size_t)(g.x0 + t);
         sfor<W>([&](auto Y) {
             constexpr int y = decltype(Y)::value;
-            z[y] = make_float2(load_px<DT>(p.a, idx + (size_t)y * p.pitch),
-                               load_px<DT>(p.b, idx + (size_t)y * p.pitch));
+            z[y] = make_float2(load_px<DT>(g.a, idx + (size_t)y * g.pitch),
+                               load_px<DT>(g.b, idx + (size_t)y * g.pitch));
         });
     }
     // ---- FFT along y (column t), transpose through LDS ----
@@ -144,7 +212,7 @@ __device__ __forceinline__ float forward_product(const CorrParams& p, int x0, in
     const float f = 0.25f * scale;
     // ---- separate the two real spectra and form P = B conj(A) ----
     const int m = (W - t) & (W - 1);              // row of -ky
-    const bool compat_row = p.real_compat && (t == 0 || t == H2);
+    const bool compat_row = real_compat && (t == 0 || t == H2);
     sfor<W>([&](auto K) {
         constexpr int kx = decltype(K)::value;
         constexpr int kxm = (W - kx) & (W - 1);
@@ -156,7 +224,7 @@ __device__ __forceinline__ float forward_product(const CorrParams& p, int x0, in
         float pr = f * (di * sr - dr * si);
         float pi = -f * (dr * sr + di * si);
         // fft.h:166-183 leaves ky in {0, H/2} and kx = 0 empty
-        if (compat_row || (kx == 0 && p.real_compat)) { pr = 0.f; pi = 0.f; }
+        if (compat_row || (kx == 0 && real_compat)) { pr = 0.f; pi = 0.f; }
         if constexpr (ACC) q[kx] = make_float2(q[kx].x - pi, q[kx].y + pr);
         else q[kx] = make_float2(pr, pi);
     });
@@ -199,42 +267,34 @@ __device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, i
     }
 }
 
-template <int W, int DT, bool PLANES, int ROUNDS>
-__global__ __launch_bounds__(kCorrThreads) void corr_fft_kernel(CorrParams p)
+// One wave task: ROUNDS x (64 / W) windows, windows
+// [(task * ROUNDS + r) * 64/W + slot] for round r and lane group slot.
+template <int W, int DT, bool PLANES, int ROUNDS, bool BATCH>
+__device__ __forceinline__ void corr_task(const CorrParams& p, int task, float2 (*lds)[W + 1], int lane)
 {
-    constexpr int WPW = 64 / W;                 // windows per wave and round
-    constexpr int WAVES = kCorrThreads / 64;
-    constexpr int LS = W + 1;                   // padded LDS row (float2)
+    constexpr int WPW = 64 / W;                 // lane groups per wave
     constexpr int H2 = W / 2;
-    __shared__ float2 lds_all[WAVES][WPW][W][LS];
-
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
     const int slot = lane / W;
     const int t = lane % W;
-    float2 (*lds)[LS] = lds_all[wave][slot];
 
-    // round r of this lane group handles window local[r]
-    int local[ROUNDS], x0[ROUNDS], y0[ROUNDS];
+    // round r of this lane group handles window (task * ROUNDS + r) * WPW + slot
     bool active[ROUNDS];
-#pragma unroll
-    for (int r = 0; r < ROUNDS; ++r) {
-        local[r] = ((blockIdx.x * WAVES + wave) * ROUNDS + r) * WPW + slot;
-        active[r] = local[r] < p.n_windows;
-        const int gid = p.first + (active[r] ? local[r] : p.n_windows - 1);
-        const int ix = gid % p.nx, iy = gid / p.nx;
-        x0[r] = p.x_start + ix * p.step_x;
-        y0[r] = p.y_start + iy * p.step_y;
-    }
+    WinGeom geo[ROUNDS];
+    sfor<ROUNDS>([&](auto R) {
+        constexpr int r = decltype(R)::value;
+        geo[r] = window_geom<BATCH>(p, (task * ROUNDS + r) * WPW + slot);
+        active[r] = geo[r].valid;
+    });
 
     // ---- forward transforms; with two rounds the spectra pack as
     // Q = P0 + i P1, whose inverse is c0 + i c1 (both planes are real) ----
     const int gbase = lane - t;
     float2 q[W];
     float scale = 1.f, inv_scale = 1.f;
-    const float zmax0 = forward_product<W, DT, false>(p, x0[0], y0[0], t, lds, q, 0.f, scale);
+    const bool compat = p.real_compat != 0;
+    const float zmax0 = forward_product<W, DT, false>(geo[0], compat, t, lds, q, 0.f, scale);
     if constexpr (ROUNDS == 2) {
-        forward_product<W, DT, true>(p, x0[1], y0[1], t, lds, q, zmax0, scale);
+        forward_product<W, DT, true>(geo[ROUNDS - 1], compat, t, lds, q, zmax0, scale);
         inv_scale = 1.f / scale;
     }
 
@@ -266,7 +326,7 @@ __global__ __launch_bounds__(kCorrThreads) void corr_fft_kernel(CorrParams p)
         sfor<ROUNDS>([&](auto R) {
             constexpr int r = decltype(R)::value;
             if (active[r]) {
-                double* dst = p.planes + (size_t)local[r] * W * W + t;
+                double* dst = p.planes + (size_t)geo[r].out * W * W + t;
                 sfor<W>([&](auto Hh) {
                     constexpr int h = decltype(Hh)::value;
                     dst[h * W] = (double)s[r][h];
@@ -353,11 +413,11 @@ __global__ __launch_bounds__(kCorrThreads) void corr_fft_kernel(CorrParams p)
                 if (v1[r] - v2[r] <= tol || v1[r] - nmax <= tol) flags |= OPIV_FLAG_NEAR_TIE;
                 if (v1[r] == v2[r]) flags |= OPIV_FLAG_EXACT_TIE;
                 // main.cpp:220-234
-                const int midx = x0[r] + W / 2, midy = y0[r] + W / 2;
+                const int midx = geo[r].x0 + W / 2, midy = geo[r].y0 + W / 2;
                 res.x = (double)midx;
-                res.y = (double)p.img_h - (double)midy;
-                res.u = (double)midx - ((double)x0[r] + plx);
-                res.v = (double)midy - ((double)y0[r] + ply);
+                res.y = (double)geo[r].img_h - (double)midy;
+                res.u = (double)midx - ((double)geo[r].x0 + plx);
+                res.v = (double)midy - ((double)geo[r].y0 + ply);
                 res.sn = (v2[r] > 0.f) ? (double)v1[r] / (double)v2[r] : 0.0;
                 res.peak0 = (double)v1[r];
                 res.peak1 = (double)v2[r];
@@ -365,48 +425,102 @@ __global__ __launch_bounds__(kCorrThreads) void corr_fft_kernel(CorrParams p)
                 res.py = py[r];
                 res.flags = flags;
             }
-            p.out[local[r]] = res;
+            p.out[geo[r].out] = res;
         });
     }
 }
 
-template <int W, int DT>
-static hipError_t launch_w(const CorrParams& p, bool planes, hipStream_t stream)
+// occupancy target per window size: the VGPR budget that keeps the
+// register-resident lines (nearly) without spilling; batch kernels carry
+// per-lane frame pointers and get a little more room at W = 16
+template <int W, bool BATCH>
+constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (BATCH ? 3 : 4) : (W == 32 ? 2 : 1); }
+
+// BATCH reads each window's pair from the device table (per-lane frame
+// pointers) and loops over wave tasks, so the launch may be persistent; a
+// single-pair launch has uniform geometry and one task per wave.
+template <int W, int DT, bool PLANES, int ROUNDS, int NT, bool BATCH>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves_per_simd<W, BATCH>())))
+void corr_fft_kernel(CorrParams p)
 {
-    // two windows per lane group (packed inverse) while the second set of
-    // spectra fits in registers
-    constexpr int ROUNDS = W <= 32 ? 2 : 1;
-    constexpr int per_block = (kCorrThreads / 64) * ROUNDS * (64 / W);
-    const int blocks = (p.n_windows + per_block - 1) / per_block;
-    if (planes)
-        hipLaunchKernelGGL((corr_fft_kernel<W, DT, true, ROUNDS>), dim3(blocks), dim3(kCorrThreads), 0, stream, p);
-    else
-        hipLaunchKernelGGL((corr_fft_kernel<W, DT, false, ROUNDS>), dim3(blocks), dim3(kCorrThreads), 0, stream, p);
+    constexpr int WPW = 64 / W;
+    constexpr int WAVES = NT / 64;
+    __shared__ float2 lds_all[WAVES][WPW][W][W + 1];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int ntasks = (p.n_windows + ROUNDS * WPW - 1) / (ROUNDS * WPW);
+    if constexpr (BATCH) {
+        for (int task = blockIdx.x * WAVES + wave; task < ntasks; task += gridDim.x * WAVES)
+            corr_task<W, DT, PLANES, ROUNDS, BATCH>(p, task, lds_all[wave][lane / W], lane);
+    } else {
+        const int task = blockIdx.x * WAVES + wave;
+        if (task < ntasks) corr_task<W, DT, PLANES, ROUNDS, BATCH>(p, task, lds_all[wave][lane / W], lane);
+    }
+}
+
+template <int W>
+constexpr int rounds_for() { return W <= 32 ? 2 : 1; }   // second spectrum fits in VGPRs up to 32
+
+template <int W, int DT, int NT, bool BATCH>
+static hipError_t launch_wn(const CorrParams& p, bool planes, int blocks, hipStream_t stream)
+{
+    constexpr int ROUNDS = rounds_for<W>();
+    constexpr int tasks_per_block = NT / 64;
+    const int ntasks = (p.n_windows + ROUNDS * (64 / W) - 1) / (ROUNDS * (64 / W));
+    const int full = (ntasks + tasks_per_block - 1) / tasks_per_block;
+    const int grid = blocks > 0 && blocks < full ? blocks : full;
+    if constexpr (BATCH) {
+        hipLaunchKernelGGL((corr_fft_kernel<W, DT, false, ROUNDS, NT, true>), dim3(grid), dim3(NT), 0, stream, p);
+    } else {
+        if (planes)
+            hipLaunchKernelGGL((corr_fft_kernel<W, DT, true, ROUNDS, NT, false>), dim3(grid), dim3(NT), 0, stream, p);
+        else
+            hipLaunchKernelGGL((corr_fft_kernel<W, DT, false, ROUNDS, NT, false>), dim3(grid), dim3(NT), 0, stream, p);
+    }
     return hipGetLastError();
 }
 
+template <int W, int DT>
+static hipError_t launch_w(const CorrParams& p, bool planes, int nt, int blocks, hipStream_t stream)
+{
+    if (p.pairs == nullptr) {
+        if (nt != 256) return hipErrorInvalidValue;
+        return launch_wn<W, DT, 256, false>(p, planes, 0, stream);   // one task per wave
+    }
+    if (planes) return hipErrorInvalidValue;
+    if (nt == 256) return launch_wn<W, DT, 256, true>(p, false, blocks, stream);
+    if (nt == 64) return launch_wn<W, DT, 64, true>(p, false, blocks, stream);
+    return hipErrorInvalidValue;
+}
+
 template <int DT>
-static hipError_t launch_dt(const CorrParams& p, int win, bool planes, hipStream_t stream)
+static hipError_t launch_dt(const CorrParams& p, int win, bool planes, int nt, int blocks, hipStream_t stream)
 {
     switch (win) {
-    case 8: return launch_w<8, DT>(p, planes, stream);
-    case 16: return launch_w<16, DT>(p, planes, stream);
-    case 32: return launch_w<32, DT>(p, planes, stream);
-    case 64: return launch_w<64, DT>(p, planes, stream);
+    case 8: return launch_w<8, DT>(p, planes, nt, blocks, stream);
+    case 16: return launch_w<16, DT>(p, planes, nt, blocks, stream);
+    case 32: return launch_w<32, DT>(p, planes, nt, blocks, stream);
+    case 64: return launch_w<64, DT>(p, planes, nt, blocks, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_corr_fft_shape(const CorrParams& p, int win, int dtype, bool planes, int nt, int blocks,
+                                 hipStream_t stream)
+{
+    if (p.n_windows <= 0) return hipSuccess;
+    switch (dtype) {
+    case 0: return launch_dt<0>(p, win, planes, nt, blocks, stream);
+    case 1: return launch_dt<1>(p, win, planes, nt, blocks, stream);
+    case 2: return launch_dt<2>(p, win, planes, nt, blocks, stream);
+    case 3: return launch_dt<3>(p, win, planes, nt, blocks, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_corr_fft(const CorrParams& p, int win, int dtype, bool planes, hipStream_t stream)
 {
-    if (p.n_windows <= 0) return hipSuccess;
-    switch (dtype) {
-    case 0: return launch_dt<0>(p, win, planes, stream);
-    case 1: return launch_dt<1>(p, win, planes, stream);
-    case 2: return launch_dt<2>(p, win, planes, stream);
-    case 3: return launch_dt<3>(p, win, planes, stream);
-    default: return hipErrorInvalidValue;
-    }
+    return launch_corr_fft_shape(p, win, dtype, planes, 256, 0, stream);
 }
 
 } // namespace opiv

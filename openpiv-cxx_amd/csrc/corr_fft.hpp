@@ -10,7 +10,27 @@
 
 namespace opiv {
 
-constexpr int kCorrThreads = 256;
+// One frame pair of a batch pass (device table): its frames and its grid.
+// The pass's windows are numbered pair by pair, pair k owning
+// [first, first + nx * ny); each pair's range starts on a multiple of the
+// kernel's task size (batch_align), so every window shares its packed
+// inverse FFT with the same partner as in a single-pair launch.  Results of
+// pair k start at out_first.
+struct BatchPair {
+    const void* a;
+    const void* b;
+    size_t pitch;
+    int32_t x_start, y_start;
+    int32_t step_x, step_y;
+    int32_t nx, ny;
+    uint32_t img_h;
+    int32_t first;
+    int32_t out_first;
+    int32_t pad_;
+};
+
+// windows per wave task of the kernel for window size win
+constexpr int batch_align(int win) { return (win <= 32 ? 2 : 1) * (64 / win); }
 
 // One launch processes windows [first, first + n_windows) of a cartesian
 // grid (grid.impl.h:44-58): window g sits at
@@ -30,8 +50,17 @@ struct CorrParams {
     float tie_tol;           // relative FP32 tie tolerance for OPIV_FLAG_NEAR_TIE
     opiv_window_result* out; // fused mode: results indexed by local window
     double* planes;          // plane mode: swapped planes indexed by local window
+    const BatchPair* pairs;  // batch mode (fused only): pair table, else nullptr
+    int32_t npairs;
 };
 
+// Default shape: 256 threads per block, one block per 4 wave tasks.
 hipError_t launch_corr_fft(const CorrParams& p, int win, int dtype, bool planes, hipStream_t stream);
+
+// Explicit shape: nt threads per block (256; batch launches also 64) and at
+// most `blocks` blocks (0 = one per task group; fewer makes the launch
+// persistent).  The shape never changes the arithmetic of a window.
+hipError_t launch_corr_fft_shape(const CorrParams& p, int win, int dtype, bool planes, int nt, int blocks,
+                                 hipStream_t stream);
 
 } // namespace opiv

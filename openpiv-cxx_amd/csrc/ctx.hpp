@@ -1,0 +1,109 @@
+// ctx.hpp -- host-side internals shared by the C ABI translation units:
+// the per-thread error text, status helpers, growable device buffers and the
+// context and plan objects behind the opaque handles of openpiv_amd.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "openpiv_amd.h"
+
+namespace opiv::detail {
+
+inline thread_local std::string g_last_error;
+
+inline int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+inline int hip_fail(hipError_t e, const char* what)
+{
+    return fail(OPIV_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(call)                                        \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);    \
+    } while (0)
+
+inline bool is_pow2(uint64_t v) { return v && (v == (v & (~v + 1))); }   // util.h:67-70
+
+inline size_t dtype_size(int dt)
+{
+    switch (dt) {
+    case OPIV_U8: return 1;
+    case OPIV_U16: return 2;
+    case OPIV_F32: return 4;
+    case OPIV_F64: return 8;
+    default: return 0;
+    }
+}
+
+// relative gap under which two FP32 correlation values count as tied for
+// OPIV_FLAG_NEAR_TIE
+constexpr float kTieTol = 2e-6f;
+
+// device buffer that grows on demand
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+inline bool is_device_ptr(const void* ptr)
+{
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof(attr));
+    hipError_t e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();   // unregistered host memory: clear the sticky error
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+} // namespace opiv::detail
+
+struct opiv_ctx {
+    int device = 0;
+    int cu_count = 0;
+    std::string arch;
+    hipStream_t stream = nullptr;
+    opiv::detail::DevBuf stage_a, stage_b, stage_out;
+};
+
+struct opiv_plan {
+    opiv_ctx* ctx = nullptr;
+    opiv_plan_desc desc{};
+    uint32_t off_x = 0, off_y = 0;
+    uint32_t nx = 0, ny = 0;
+    int32_t x_start = 0, y_start = 0;
+};

@@ -52,6 +52,8 @@ EXPORTS = (
     "opiv_plan_create", "opiv_plan_destroy", "opiv_plan_window_count", "opiv_plan_grid",
     "opiv_run_pair", "opiv_run_pair_async",
     "opiv_correlate_windows", "opiv_auto_correlate_windows",
+    "opiv_batch_create", "opiv_batch_destroy", "opiv_batch_window_count", "opiv_batch_offsets",
+    "opiv_batch_set_pairs", "opiv_batch_shape", "opiv_batch_run",
 )
 
 
@@ -72,6 +74,17 @@ class PlanDesc(ctypes.Structure):
         ("overlap", ctypes.c_double),
         ("variant", ctypes.c_int32), ("limit_search", ctypes.c_int32),
     ]
+
+
+class PairDesc(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p),
+                ("img_w", ctypes.c_uint32), ("img_h", ctypes.c_uint32), ("pitch", ctypes.c_size_t)]
+
+
+class BatchDesc(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("npasses", ctypes.c_uint32),
+                ("win", ctypes.c_uint32 * 4), ("overlap", ctypes.c_double * 4),
+                ("variant", ctypes.c_int32), ("limit_search", ctypes.c_int32)]
 
 
 class SynthDesc(ctypes.Structure):
@@ -121,6 +134,14 @@ def load() -> ctypes.CDLL:
     lib.opiv_run_pair_async.argtypes = [vp, vp, vp, vp, sz]
     lib.opiv_correlate_windows.argtypes = [vp, u32, sz, i32, vp, vp, vp]
     lib.opiv_auto_correlate_windows.argtypes = [vp, u32, sz, vp, vp]
+    lib.opiv_batch_create.argtypes = [vp, ctypes.POINTER(BatchDesc), ctypes.POINTER(PairDesc), sz,
+                                      ctypes.POINTER(vp)]
+    lib.opiv_batch_destroy.argtypes = [vp]
+    lib.opiv_batch_window_count.argtypes = [vp, ctypes.POINTER(sz)]
+    lib.opiv_batch_offsets.argtypes = [vp, vp]
+    lib.opiv_batch_set_pairs.argtypes = [vp, ctypes.POINTER(PairDesc), sz]
+    lib.opiv_batch_shape.argtypes = [vp, u32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    lib.opiv_batch_run.argtypes = [vp, vp, sz]
     for name in EXPORTS:
         if name not in ("opiv_last_error", "opiv_version", "opiv_ctx_stream"):
             getattr(lib, name).restype = ctypes.c_int
@@ -279,6 +300,73 @@ class Plan:
     def run_async(self, a_ptr: int, b_ptr: int, out_ptr: int) -> None:
         """Enqueue on the context stream; device pointers only."""
         _check(self._lib.opiv_run_pair_async(self.handle, a_ptr, b_ptr, out_ptr, self.count))
+
+
+class Batch:
+    """Many frame pairs x several independent window passes, one graph launch
+    per run (opiv_batch).  ``pairs`` is a list of (a_ptr, b_ptr, width, height)
+    device frames of one dtype; results of (pass p, pair i) start at
+    ``offsets[p * npairs + i]``."""
+
+    def __init__(self, ctx: Context, pairs, wins=(64, 32, 16), overlaps=None, *, dtype=np.float32,
+                 variant: int = CIRCULAR_FFT_FP32, limit_search: bool = False):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        overlaps = overlaps or [0.5] * len(wins)
+        d = BatchDesc()
+        d.dtype = DTYPE_CODES[np.dtype(dtype)]
+        d.npasses = len(wins)
+        for i, (w, o) in enumerate(zip(wins, overlaps)):
+            d.win[i] = w
+            d.overlap[i] = o
+        d.variant = variant
+        d.limit_search = 1 if limit_search else 0
+        self.npairs = len(pairs)
+        arr = self._pairs(pairs)
+        h = ctypes.c_void_p()
+        _check(self._lib.opiv_batch_create(ctx.handle, ctypes.byref(d), arr, self.npairs, ctypes.byref(h)))
+        self.handle = h
+        n = ctypes.c_size_t()
+        _check(self._lib.opiv_batch_window_count(h, ctypes.byref(n)))
+        self.count = n.value
+        off = np.zeros(len(wins) * self.npairs + 1, dtype=np.uint64)
+        _check(self._lib.opiv_batch_offsets(h, off.ctypes.data))
+        self.offsets = off.astype(np.int64)
+        self.npasses = len(wins)
+
+    @staticmethod
+    def _pairs(pairs):
+        arr = (PairDesc * len(pairs))()
+        for i, (a, b, w, h) in enumerate(pairs):
+            arr[i].a, arr[i].b, arr[i].img_w, arr[i].img_h, arr[i].pitch = _ptr(a), _ptr(b), w, h, 0
+        return arr
+
+    def set_pairs(self, pairs) -> None:
+        _check(self._lib.opiv_batch_set_pairs(self.handle, self._pairs(pairs), len(pairs)))
+
+    def shape(self, p: int) -> tuple[int, int]:
+        t, b = ctypes.c_int32(), ctypes.c_int32()
+        _check(self._lib.opiv_batch_shape(self.handle, p, ctypes.byref(t), ctypes.byref(b)))
+        return t.value, b.value
+
+    def run(self, out_ptr: int) -> None:
+        """Enqueue all passes (one graph launch); out_ptr: device, ``count`` results."""
+        _check(self._lib.opiv_batch_run(self.handle, out_ptr, self.count))
+
+    def block(self, results: np.ndarray, p: int, i: int) -> np.ndarray:
+        k = p * self.npairs + i
+        return results[self.offsets[k]:self.offsets[k + 1]]
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.opiv_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- device memory

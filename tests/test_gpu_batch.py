@@ -1,0 +1,135 @@
+"""Batches: many uneven frame pairs x passes 64 -> 32 -> 16 in one graph
+launch.  Every window of the batch must be bit-identical to the single-pair
+path (same kernel arithmetic, only the addressing differs), whatever launch
+shape the tuner picks; the single-pair path itself is held to the oracle bar
+in test_gpu_parity.  A sample of pairs is also checked against the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import openpiv_amd as oa
+import oracle as orc
+from test_gpu_parity import compare
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(512, 512), (1280, 800), (384, 256), (1024, 1024), (700, 420), (256, 256)]
+WINS = (64, 32, 16)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_pairs(dtype=np.float32, seed0=100):
+    frames = []
+    for k, (w, h) in enumerate(SIZES):
+        rng = np.random.default_rng(seed0 + k)
+        cx, cy = rng.uniform(0.3, 0.7) * w, rng.uniform(0.3, 0.7) * h
+        a, b = oa.synth_pair(w, h, seed=seed0 + k, field=oa.FIELD_LAMB_OSEEN, params=(cx, cy, 40.0, 5.0),
+                             dtype=dtype)
+        frames.append((a, b))
+    return frames
+
+
+@pytest.fixture(scope="module")
+def batch_setup(gpu_ctx):
+    frames = make_pairs()
+    bufs = [(oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)) for a, b in frames]
+    pairs = [(da.ptr, db.ptr, a.shape[1], a.shape[0]) for (a, b), (da, db) in zip(frames, bufs)]
+    return frames, bufs, pairs
+
+
+def eager(ctx, frames, win):
+    out = []
+    for a, b in frames:
+        plan = oa.Plan(ctx, a.shape[1], a.shape[0], win, overlap=0.5, dtype=a.dtype)
+        out.append(plan.run(a, b))
+    return out
+
+
+def run_batch(batch):
+    dout = oa.DeviceBuffer(batch.count * oa.RESULT_DTYPE.itemsize)
+    batch.run(dout.ptr)
+    batch.ctx.synchronize()
+    return dout, dout.download(np.zeros(batch.count, oa.RESULT_DTYPE))
+
+
+def test_batch_equals_single_pair_path(gpu_ctx, batch_setup):
+    frames, bufs, pairs = batch_setup
+    say("batch vs single-pair")
+    batch = oa.Batch(gpu_ctx, pairs, WINS)
+    dout, res = run_batch(batch)
+    for p, win in enumerate(WINS):
+        ref = eager(gpu_ctx, frames, win)
+        for i in range(len(frames)):
+            assert batch.block(res, p, i).tobytes() == ref[i].tobytes(), (win, i)
+    say("shapes", [batch.shape(p) for p in range(len(WINS))])
+
+
+def test_batch_against_oracle_sample(gpu_ctx, batch_setup):
+    frames, bufs, pairs = batch_setup
+    batch = oa.Batch(gpu_ctx, pairs, WINS)
+    _, res = run_batch(batch)
+    for p, win in enumerate(WINS):
+        for i in (1, 4):
+            a, b = frames[i]
+            ref = orc.process_pair(a, b, win, win // 2, win // 2, threads=16)
+            compare(batch.block(res, p, i), ref, f"batch-w{win}-pair{i}")
+
+
+def test_graph_replays_with_eager_work_between(gpu_ctx, batch_setup):
+    frames, bufs, pairs = batch_setup
+    batch = oa.Batch(gpu_ctx, pairs, WINS)
+    dout = oa.DeviceBuffer(batch.count * oa.RESULT_DTYPE.itemsize)
+    first = None
+    plan = oa.Plan(gpu_ctx, 512, 512, 32, dtype=np.float32)
+    for k in range(4):
+        dout.zero()
+        batch.run(dout.ptr)
+        gpu_ctx.synchronize()
+        got = dout.download(np.zeros(batch.count, oa.RESULT_DTYPE))
+        first = got if first is None else first
+        assert got.tobytes() == first.tobytes(), k
+        plan.run(frames[0][0], frames[0][1])          # eager work between replays
+
+
+def test_set_pairs_swaps_frames(gpu_ctx, batch_setup):
+    frames, bufs, pairs = batch_setup
+    batch = oa.Batch(gpu_ctx, pairs, (32,))
+    swapped = [(pb, pa, w, h) for (pa, pb, w, h) in pairs]
+    batch.set_pairs(swapped)
+    _, res = run_batch(batch)
+    for i, (a, b) in enumerate(frames):
+        plan = oa.Plan(gpu_ctx, a.shape[1], a.shape[0], 32, dtype=a.dtype)
+        assert batch.block(res, 0, i).tobytes() == plan.run(b, a).tobytes()
+    with pytest.raises(oa.OpivError):
+        batch.set_pairs(swapped[:-1])
+
+
+def test_results_do_not_depend_on_launch_shape(gpu_ctx, batch_setup, tmp_path, monkeypatch):
+    frames, bufs, pairs = batch_setup
+    outs = []
+    for nt, bpc4 in ((256, 0), (64, 4), (256, 8), (64, 64)):
+        cache = tmp_path / f"tune_{nt}_{bpc4}.txt"
+        n32 = sum((1 + (w - 32) // 16) * (1 + (h - 32) // 16) for w, h in SIZES)
+        key = f"gfx950 32 2 {int(round(np.log2(n32)))}"
+        cache.write_text(f"{key} {nt} {bpc4}\n")
+        monkeypatch.setenv("OPIV_TUNE_CACHE", str(cache))
+        monkeypatch.setenv("OPIV_TUNE", "0")
+        batch = oa.Batch(gpu_ctx, pairs, (32,))
+        t, blocks = batch.shape(0)
+        assert t == nt, (t, nt)
+        outs.append(run_batch(batch)[1])
+        say(f"shape nt={t} blocks={blocks}")
+    for o in outs[1:]:
+        assert o.tobytes() == outs[0].tobytes()
+
+
+def test_batch_errors(gpu_ctx, batch_setup):
+    frames, bufs, pairs = batch_setup
+    with pytest.raises(oa.OpivError, match="power of 2"):
+        oa.Batch(gpu_ctx, pairs, (24,))
+    with pytest.raises(oa.OpivError, match="bigger than image"):
+        oa.Batch(gpu_ctx, [(pairs[0][0], pairs[0][1], 40, 40)], (64,))
