@@ -142,6 +142,51 @@ def copy_calibration(mib: int, reps: int):
     print(json.dumps({"copy_calibration_mib": mib, "reps": reps}), flush=True)
 
 
+def bench_adjoint(steps: int, warmup: int):
+    """1024^2 f32 pair, uniform shift (1.6, -0.8), 32^2 windows at step 16:
+    device time of the forward pass, of the VJP (both frames' gradients of
+    sum cot . (u, v)) and of the variance propagation (sigma = 2)."""
+    import openpiv_amd as oa
+    a, b = oa.synth_pair(1024, 1024, seed=5, params=(1.6, -0.8), dtype=np.float32)
+    ctx = oa.Context(0)
+    plan = oa.Plan(ctx, 1024, 1024, 32, dtype=np.float32)
+    n = plan.count
+    da, db = oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)
+    dfwd = oa.DeviceBuffer(n * RESULT_BYTES)
+    dcot = oa.DeviceBuffer.from_array(np.random.default_rng(0).normal(size=(n, 2)))
+    dga, dgb = oa.DeviceBuffer(1024 * 1024 * 4), oa.DeviceBuffer(1024 * 1024 * 4)
+    dvar = oa.DeviceBuffer(n * 16)
+    lib = oa.load()
+    ev = HipEvents()
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        ctx.synchronize()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, ctx.stream)
+        for _ in range(steps):
+            fn()
+        ev.record(e1, ctx.stream)
+        ctx.synchronize()
+        return ev.elapsed_ms(e0, e1) / steps
+
+    fwd_ms = timed(lambda: plan.run_async(da.ptr, db.ptr, dfwd.ptr))
+    vjp_ms = timed(lambda: oa._check(lib.opiv_vjp_pair(plan.handle, da.ptr, db.ptr, dfwd.ptr, dcot.ptr,
+                                                        dga.ptr, dgb.ptr)))
+    var_ms = timed(lambda: oa._check(lib.opiv_variance_pair(plan.handle, da.ptr, db.ptr, dfwd.ptr, 2.0,
+                                                             dvar.ptr)))
+    line = {"metric": "adjoint cost relative to the forward pass (1024^2 f32, 32^2 windows, step 16)",
+            "value": round(vjp_ms / fwd_ms, 3), "unit": "x forward", "n_gpus": 1, "steps": steps,
+            "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "1024x1024 f32 pair, uniform shift (1.6, -0.8), 32x32 windows step 16",
+                       "windows": n},
+            "forward_ms": round(fwd_ms, 4), "vjp_ms": round(vjp_ms, 4), "variance_ms": round(var_ms, 4),
+            "variance_over_forward": round(var_ms / fwd_ms, 3),
+            "note": "vjp/variance times include their stream synchronisation and (vjp) the gradient clears"}
+    print(json.dumps(line), flush=True)
+
+
 BATCH_SIZES = [(512, 512), (1024, 1024), (2048, 2048), (4096, 4096), (1280, 800)]
 
 
@@ -214,6 +259,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--win", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="1024^2 f32 differentiation workload: forward vs VJP vs variance (prints its own line)")
     ap.add_argument("--batch", action="store_true",
                     help="24 uneven frame pairs x passes 64/32/16 in one graph launch (prints its own line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -231,6 +278,9 @@ def main():
         return
     if args.batch:
         bench_batch(args.steps, args.warmup)
+        return
+    if args.adjoint:
+        bench_adjoint(args.steps, args.warmup)
         return
 
     win = args.win

@@ -143,6 +143,28 @@ int opiv_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t varian
 int opiv_auto_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n,
                                 const double* a, double* planes);
 
+/* ---- adjoint: displacement sensitivities (no reference counterpart) ----
+ *
+ * Derivatives of each window's vector (u, v) of main.cpp:227 with respect to
+ * the pixels of both frames, with the integer peak held fixed.  The adjoint
+ * of the circular correlation at the five stencil shifts is again a
+ * correlation; the fit's derivatives use FP64 logs of stencil values
+ * recomputed as direct dot products.  Only OPIV_CIRCULAR_FFT_FP32 plans.
+ * fwd: the plan's results of opiv_run_pair for the same frames (NULL = run
+ * the forward pass here).  Windows flagged FEWER_THAN_2_PEAKS or
+ * NONFINITE_FIT contribute nothing.  All pointers host or device; returns
+ * when the outputs are written. */
+
+/* grad_a, grad_b (frame-sized, pitch * img_h floats) =
+ * sum_i cot[2i] * du_i/d(frame) + cot[2i+1] * dv_i/d(frame) */
+int opiv_vjp_pair(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,
+                  const double* cot, float* grad_a, float* grad_b);
+
+/* var[2i], var[2i+1] = sigma^2 * |d(u_i, v_i)/d(both frames)|^2: the
+ * variance of (u_i, v_i) under i.i.d. pixel noise sigma (linear propagation) */
+int opiv_variance_pair(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,
+                       double sigma, double* var);
+
 /* ---- batches: many frame pairs and several window passes per launch ----
  *
  * The reference processes one pair and one window size per run of

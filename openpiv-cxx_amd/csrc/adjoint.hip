@@ -1,0 +1,280 @@
+// adjoint.hip -- vector-Jacobian products of each window's displacement
+// (u, v) with respect to both frames' pixels, and the per-window variance of
+// (u, v) under i.i.d. pixel noise.  No counterpart in the reference; it
+// differentiates the reference's vector (examples/process/main.cpp:220-227).
+//
+// For a window with integer peak (px, py) (held fixed: the argmax has zero
+// derivative almost everywhere), u = W/2 - px - dx and
+// dx = (ln l - ln r) / (2 (ln l + ln r - 2 ln c)) (image_utils.impl.h:86-94),
+// where l, c, r (and up, down for v) are circular correlations
+// C[s] = sum_n a[n] b[n+s] at five fixed shifts s_k.  Hence
+//   du/da[n] = sum_k du/dC_k * b[n + s_k],   du/db[m] = sum_k du/dC_k * a[m - s_k]
+// -- the adjoint of a correlation is again a (5-tap) correlation.
+//
+// Mapping: W lanes own a window (lane t = column t, as in the forward
+// kernel).  Both windows go to LDS; the taps of a lane touch three columns
+// per frame, which it caches one at a time, rotated by the peak shift so
+// every tap is a register read.  The five C_k are recomputed as direct dot
+// products with FP64 accumulation, so the fit's derivatives use FP64 logs
+// of accurate values.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "adjoint.hpp"
+#include "fft_reg.hpp"
+
+namespace opiv {
+
+using fftreg::sfor;
+
+namespace {
+
+__device__ __forceinline__ void wave_sync_adj()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+
+template <int DT>
+__device__ __forceinline__ float load_px(const void* base, size_t idx)
+{
+    if constexpr (DT == 0) return (float)((gptr<uint8_t>)base)[idx];
+    else if constexpr (DT == 1) return (float)((gptr<uint16_t>)base)[idx];
+    else if constexpr (DT == 2) return ((gptr<float>)base)[idx];
+    else return (float)((gptr<double>)base)[idx];
+}
+
+// d(num/den)/d(l, c, r) with num = ln l - ln r, den = 2 (ln l + ln r - 2 ln c)
+__device__ __forceinline__ void gauss3_grad(double l, double c, double r, double& gl, double& gc, double& gr)
+{
+    const double L = log(l), R = log(r), Cc = log(c);
+    const double num = L - R, den = 2.0 * (L + R - 2.0 * Cc);
+    if (den == 0.0) {
+        gl = gc = gr = 0.0;
+        return;
+    }
+    const double d2 = den * den;
+    gl = (den - 2.0 * num) / d2 / l;
+    gr = (-den - 2.0 * num) / d2 / r;
+    gc = 4.0 * num / d2 / c;
+}
+
+} // namespace
+
+template <int W, int DT, bool VARIANCE>
+__global__ __launch_bounds__(256) void adjoint_kernel(AdjParams p)
+{
+    constexpr int WPW = 64 / W;
+    constexpr int WAVES = 256 / 64;
+    constexpr int H2 = W / 2;
+    constexpr int M = W - 1;
+    __shared__ float lds_all[WAVES][WPW][2][W][W + 1];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int slot = lane / W;
+    const int t = lane % W;
+    float (*la)[W + 1] = lds_all[wave][slot][0];
+    float (*lb)[W + 1] = lds_all[wave][slot][1];
+
+    const int local = (blockIdx.x * WAVES + wave) * WPW + slot;
+    const bool active = local < p.n_windows;
+    const int i = active ? local : p.n_windows - 1;
+    const int ix = i % p.nx, iy = i / p.nx;
+    const int x0 = p.x_start + ix * p.step_x;
+    const int y0 = p.y_start + iy * p.step_y;
+    const opiv_window_result fw = p.fwd[i];
+    const bool valid = active && !(fw.flags & (OPIV_FLAG_FEWER_THAN_2_PEAKS | OPIV_FLAG_NONFINITE_FIT));
+
+    // ---- both windows to LDS, column t of frame A kept in registers ----
+    float acol[W];
+    {
+        const size_t idx = (size_t)y0 * p.pitch + (size_t)(x0 + t);
+        sfor<W>([&](auto Y) {
+            constexpr int y = decltype(Y)::value;
+            acol[y] = load_px<DT>(p.a, idx + (size_t)y * p.pitch);
+            la[y][t] = acol[y];
+            lb[y][t] = load_px<DT>(p.b, idx + (size_t)y * p.pitch);
+        });
+    }
+    wave_sync_adj();
+
+    // peak shift in correlation coordinates: swapped (h, w) holds
+    // C[(h + W/2) % W][(w + W/2) % W]
+    const int sx0 = (fw.px + H2) & M, sy0 = (fw.py + H2) & M;
+
+    // taps k = l, c, r, up, down at shifts s_k = (sy0 + dy, sx0 + dx) with
+    // (dy, dx) = (0,-1), (0,0), (0,1), (-1,0), (1,0).  The taps touch three
+    // columns per frame; one column is cached at a time, rotated so that
+    // col[j] = frame[(j + rotation) % W][column] and every tap row is a
+    // compile-time register index.
+    constexpr int TDY[5] = {0, 0, 0, -1, 1};
+    constexpr int TDX[5] = {-1, 0, 1, 0, 0};
+
+    // ---- the five stencil correlations C_k = sum_n a[n] b[n + s_k] (FP64 sums) ----
+    double ck[5] = {0, 0, 0, 0, 0};
+    sfor<3>([&](auto D) {
+        constexpr int d = decltype(D)::value;          // dx = d - 1
+        const int col = (t + sx0 + d - 1) & M;
+        float bc[W + 2];                                // bc[j] = b[(j + sy0 - 1) % W][col]
+        sfor<W + 2>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            bc[j] = lb[(j + sy0 - 1) & M][col];
+        });
+        sfor<5>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if constexpr (TDX[k] == d - 1) {
+                double s = ck[k];
+                sfor<W>([&](auto Y) {
+                    constexpr int y = decltype(Y)::value;
+                    s = fma((double)acol[y], (double)bc[y + TDY[k] + 1], s);
+                });
+                ck[k] = s;
+            }
+        });
+    });
+#pragma unroll
+    for (int off = W / 2; off >= 1; off >>= 1) {
+        sfor<5>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            ck[k] += __shfl_xor(ck[k], off, 64);
+        });
+    }
+
+    // ---- du/dC_k and dv/dC_k (u = W/2 - px - dx, v = W/2 - py - dy) ----
+    double gu[5] = {0, 0, 0, 0, 0}, gv[5] = {0, 0, 0, 0, 0};
+    if (valid) {
+        double gl, gcx, gr, gt, gcy, gbt;
+        gauss3_grad(ck[0], ck[1], ck[2], gl, gcx, gr);
+        gauss3_grad(ck[3], ck[1], ck[4], gt, gcy, gbt);
+        gu[0] = -gl;
+        gu[1] = -gcx;
+        gu[2] = -gr;
+        gv[1] = -gcy;
+        gv[3] = -gt;
+        gv[4] = -gbt;
+    }
+    float fu[5], fv[5];
+    {
+        const double cu = (!VARIANCE && active) ? p.cot[2 * (size_t)i] : 1.0;
+        const double cv = (!VARIANCE && active) ? p.cot[2 * (size_t)i + 1] : 1.0;
+        sfor<5>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if constexpr (VARIANCE) {
+                fu[k] = (float)gu[k];
+                fv[k] = (float)gv[k];
+            } else {
+                fu[k] = (float)(cu * gu[k] + cv * gv[k]);   // the VJP folds both cotangents
+                fv[k] = 0.f;
+            }
+        });
+    }
+
+    // ---- gradients: dC_k/da[n] = b[n + s_k], dC_k/db[m] = a[m - s_k] ----
+    // g[0][y] (frame A) and g[1][y] (frame B) of column t, for u (or the
+    // VJP) and, with VARIANCE, for v
+    double su = 0.0, sv = 0.0;
+    sfor<2>([&](auto F) {
+        constexpr int fr = decltype(F)::value;       // 0: d/da uses frame B taps, 1: d/db uses frame A
+        float g_u[W], g_v[W];
+        sfor<W>([&](auto Y) {
+            constexpr int y = decltype(Y)::value;
+            g_u[y] = 0.f;
+            g_v[y] = 0.f;
+        });
+        sfor<3>([&](auto D) {
+            constexpr int d = decltype(D)::value;
+            // frame B column t + sx0 + dx rotated by sy0 - 1, or frame A
+            // column t - sx0 - dx rotated by -sy0 - 1
+            const int col = fr == 0 ? ((t + sx0 + d - 1) & M) : ((t - sx0 - (d - 1)) & M);
+            const int rot = fr == 0 ? (sy0 - 1) : (-sy0 - 1);
+            float (*src)[W + 1] = fr == 0 ? lb : la;
+            float cc[W + 2];
+            sfor<W + 2>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                cc[j] = src[(j + rot) & M][col];
+            });
+            sfor<5>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                if constexpr (TDX[k] == d - 1) {
+                    sfor<W>([&](auto Y) {
+                        constexpr int y = decltype(Y)::value;
+                        constexpr int row = fr == 0 ? (y + TDY[k] + 1) : (y - TDY[k] + 1);
+                        g_u[y] = fmaf(fu[k], cc[row], g_u[y]);
+                        if constexpr (VARIANCE) g_v[y] = fmaf(fv[k], cc[row], g_v[y]);
+                    });
+                }
+            });
+        });
+        if constexpr (VARIANCE) {
+            sfor<W>([&](auto Y) {
+                constexpr int y = decltype(Y)::value;
+                su = fma((double)g_u[y], (double)g_u[y], su);
+                sv = fma((double)g_v[y], (double)g_v[y], sv);
+            });
+        } else if (valid) {
+            float* g = (fr == 0 ? p.grad_a : p.grad_b) + (size_t)y0 * p.gpitch + (size_t)(x0 + t);
+            sfor<W>([&](auto Y) {
+                constexpr int y = decltype(Y)::value;
+                atomicAdd(g + (size_t)y * p.gpitch, g_u[y]);
+            });
+        }
+    });
+
+    if constexpr (VARIANCE) {
+#pragma unroll
+        for (int off = W / 2; off >= 1; off >>= 1) {
+            su += __shfl_xor(su, off, 64);
+            sv += __shfl_xor(sv, off, 64);
+        }
+        if (active && t == 0) {
+            const double s2 = p.sigma * p.sigma;
+            p.var[2 * (size_t)local] = valid ? s2 * su : 0.0;
+            p.var[2 * (size_t)local + 1] = valid ? s2 * sv : 0.0;
+        }
+    }
+}
+
+template <int W, int DT>
+static hipError_t launch_adj_w(const AdjParams& p, bool variance, hipStream_t stream)
+{
+    constexpr int per_block = 4 * (64 / W);
+    const int blocks = (p.n_windows + per_block - 1) / per_block;
+    if (variance)
+        hipLaunchKernelGGL((adjoint_kernel<W, DT, true>), dim3(blocks), dim3(256), 0, stream, p);
+    else
+        hipLaunchKernelGGL((adjoint_kernel<W, DT, false>), dim3(blocks), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+
+template <int DT>
+static hipError_t launch_adj_dt(const AdjParams& p, int win, bool variance, hipStream_t stream)
+{
+    switch (win) {
+    case 8: return launch_adj_w<8, DT>(p, variance, stream);
+    case 16: return launch_adj_w<16, DT>(p, variance, stream);
+    case 32: return launch_adj_w<32, DT>(p, variance, stream);
+    case 64: return launch_adj_w<64, DT>(p, variance, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_adjoint(const AdjParams& p, int win, int dtype, bool variance, hipStream_t stream)
+{
+    if (p.n_windows <= 0) return hipSuccess;
+    switch (dtype) {
+    case 0: return launch_adj_dt<0>(p, win, variance, stream);
+    case 1: return launch_adj_dt<1>(p, win, variance, stream);
+    case 2: return launch_adj_dt<2>(p, win, variance, stream);
+    case 3: return launch_adj_dt<3>(p, win, variance, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace opiv

@@ -1,0 +1,34 @@
+// adjoint.hpp -- launch interface of the displacement adjoint kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "openpiv_amd.h"
+
+namespace opiv {
+
+struct AdjParams {
+    const void* a;
+    const void* b;
+    size_t pitch;                    // elements per frame row
+    int32_t x_start, y_start;
+    int32_t step_x, step_y;
+    int32_t nx;
+    int32_t n_windows;
+    const opiv_window_result* fwd;   // forward results (integer peaks, flags), device
+    const double* cot;               // VJP: (du, dv) cotangent per window, device
+    float* grad_a;                   // VJP: frame-sized gradients (accumulated), device
+    float* grad_b;
+    size_t gpitch;                   // elements per gradient row
+    double sigma;                    // variance: pixel noise
+    double* var;                     // variance: (var u, var v) per window, device
+};
+
+// variance = false: grad_a/grad_b += sum_i cot_i . d(u_i, v_i)/d(frames)
+// variance = true:  var_i = sigma^2 * |d(u_i, v_i)/d(frames)|^2
+hipError_t launch_adjoint(const AdjParams& p, int win, int dtype, bool variance, hipStream_t stream);
+
+} // namespace opiv

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <string>
 
+#include "adjoint.hpp"
 #include "corr_fft.hpp"
 #include "ctx.hpp"
 #include "openpiv_amd.h"
@@ -57,9 +58,9 @@ int opiv_ctx_destroy(opiv_ctx* ctx)
     if (!ctx) return OPIV_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    ctx->stage_a.release();
-    ctx->stage_b.release();
-    ctx->stage_out.release();
+    for (auto* b : { &ctx->stage_a, &ctx->stage_b, &ctx->stage_out, &ctx->stage_fwd, &ctx->stage_cot,
+                     &ctx->stage_ga, &ctx->stage_gb, &ctx->stage_var })
+        b->release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return OPIV_OK;
@@ -292,6 +293,116 @@ static int correlate_stacked(opiv_ctx* ctx, uint32_t win, size_t n, int variant,
     p.planes = dplanes;
     HIP_TRY(opiv::launch_corr_fft(p, (int)win, OPIV_F64, true, ctx->stream));
     if (!out_dev) HIP_TRY(hipMemcpyAsync(planes, dplanes, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return OPIV_OK;
+}
+
+// device copy of a host input (or the pointer itself when already on the device)
+static int stage_in(opiv_ctx* ctx, DevBuf& buf, const void* src, size_t bytes, const void** dev)
+{
+    if (is_device_ptr(src)) {
+        *dev = src;
+        return OPIV_OK;
+    }
+    HIP_TRY(buf.ensure(bytes));
+    HIP_TRY(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    *dev = buf.p;
+    return OPIV_OK;
+}
+
+// frames, forward results and geometry shared by the two adjoint entry points
+static int adjoint_setup(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,
+                         opiv::AdjParams& p)
+{
+    opiv_ctx* ctx = plan->ctx;
+    if (plan->desc.variant != OPIV_CIRCULAR_FFT_FP32)
+        return fail(OPIV_E_UNSUPPORTED, "the adjoint supports the circular correlation variant only");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t count = (size_t)plan->nx * plan->ny;
+    const size_t frame_bytes = plan->desc.pitch * plan->desc.img_h * dtype_size(plan->desc.dtype);
+    const void *da, *db, *dfwd;
+    int rc = stage_in(ctx, ctx->stage_a, a, frame_bytes, &da);
+    if (rc == OPIV_OK) rc = stage_in(ctx, ctx->stage_b, b, frame_bytes, &db);
+    if (rc != OPIV_OK) return rc;
+    if (fwd) {
+        rc = stage_in(ctx, ctx->stage_fwd, fwd, count * sizeof(opiv_window_result), &dfwd);
+        if (rc != OPIV_OK) return rc;
+    } else {
+        HIP_TRY(ctx->stage_fwd.ensure(count * sizeof(opiv_window_result)));
+        opiv::CorrParams cp = plan_params(plan, da, db, (opiv_window_result*)ctx->stage_fwd.p);
+        HIP_TRY(opiv::launch_corr_fft(cp, (int)plan->desc.win, plan->desc.dtype, false, ctx->stream));
+        dfwd = ctx->stage_fwd.p;
+    }
+    p = opiv::AdjParams{};
+    p.a = da;
+    p.b = db;
+    p.pitch = plan->desc.pitch;
+    p.x_start = plan->x_start;
+    p.y_start = plan->y_start;
+    p.step_x = (int32_t)plan->off_x;
+    p.step_y = (int32_t)plan->off_y;
+    p.nx = (int32_t)plan->nx;
+    p.n_windows = (int32_t)count;
+    p.fwd = (const opiv_window_result*)dfwd;
+    return OPIV_OK;
+}
+
+int opiv_vjp_pair(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,
+                  const double* cot, float* grad_a, float* grad_b)
+{
+    if (!plan || !a || !b || !cot || !grad_a || !grad_b) return fail(OPIV_E_INVALID, "opiv_vjp_pair: null argument");
+    opiv_ctx* ctx = plan->ctx;
+    opiv::AdjParams p;
+    int rc = adjoint_setup(plan, a, b, fwd, p);
+    if (rc != OPIV_OK) return rc;
+    const size_t count = (size_t)p.n_windows;
+    const void* dcot;
+    rc = stage_in(ctx, ctx->stage_cot, cot, count * 2 * sizeof(double), &dcot);
+    if (rc != OPIV_OK) return rc;
+    const size_t gbytes = plan->desc.pitch * plan->desc.img_h * sizeof(float);
+    const bool ga_dev = is_device_ptr(grad_a), gb_dev = is_device_ptr(grad_b);
+    float* dga = grad_a;
+    float* dgb = grad_b;
+    if (!ga_dev) {
+        HIP_TRY(ctx->stage_ga.ensure(gbytes));
+        dga = (float*)ctx->stage_ga.p;
+    }
+    if (!gb_dev) {
+        HIP_TRY(ctx->stage_gb.ensure(gbytes));
+        dgb = (float*)ctx->stage_gb.p;
+    }
+    HIP_TRY(hipMemsetAsync(dga, 0, gbytes, ctx->stream));
+    HIP_TRY(hipMemsetAsync(dgb, 0, gbytes, ctx->stream));
+    p.cot = (const double*)dcot;
+    p.grad_a = dga;
+    p.grad_b = dgb;
+    p.gpitch = plan->desc.pitch;
+    HIP_TRY(opiv::launch_adjoint(p, (int)plan->desc.win, plan->desc.dtype, false, ctx->stream));
+    if (!ga_dev) HIP_TRY(hipMemcpyAsync(grad_a, dga, gbytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (!gb_dev) HIP_TRY(hipMemcpyAsync(grad_b, dgb, gbytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return OPIV_OK;
+}
+
+int opiv_variance_pair(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,
+                       double sigma, double* var)
+{
+    if (!plan || !a || !b || !var) return fail(OPIV_E_INVALID, "opiv_variance_pair: null argument");
+    opiv_ctx* ctx = plan->ctx;
+    opiv::AdjParams p;
+    int rc = adjoint_setup(plan, a, b, fwd, p);
+    if (rc != OPIV_OK) return rc;
+    const size_t bytes = (size_t)p.n_windows * 2 * sizeof(double);
+    const bool dev = is_device_ptr(var);
+    double* dvar = var;
+    if (!dev) {
+        HIP_TRY(ctx->stage_var.ensure(bytes));
+        dvar = (double*)ctx->stage_var.p;
+    }
+    p.sigma = sigma;
+    p.var = dvar;
+    HIP_TRY(opiv::launch_adjoint(p, (int)plan->desc.win, plan->desc.dtype, true, ctx->stream));
+    if (!dev) HIP_TRY(hipMemcpyAsync(var, dvar, bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return OPIV_OK;
 }

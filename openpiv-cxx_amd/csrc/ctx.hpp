@@ -98,6 +98,7 @@ struct opiv_ctx {
     std::string arch;
     hipStream_t stream = nullptr;
     opiv::detail::DevBuf stage_a, stage_b, stage_out;
+    opiv::detail::DevBuf stage_fwd, stage_cot, stage_ga, stage_gb, stage_var;
 };
 
 struct opiv_plan {

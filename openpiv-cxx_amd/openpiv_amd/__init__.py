@@ -54,6 +54,7 @@ EXPORTS = (
     "opiv_correlate_windows", "opiv_auto_correlate_windows",
     "opiv_batch_create", "opiv_batch_destroy", "opiv_batch_window_count", "opiv_batch_offsets",
     "opiv_batch_set_pairs", "opiv_batch_shape", "opiv_batch_run",
+    "opiv_vjp_pair", "opiv_variance_pair",
 )
 
 
@@ -142,6 +143,8 @@ def load() -> ctypes.CDLL:
     lib.opiv_batch_set_pairs.argtypes = [vp, ctypes.POINTER(PairDesc), sz]
     lib.opiv_batch_shape.argtypes = [vp, u32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.opiv_batch_run.argtypes = [vp, vp, sz]
+    lib.opiv_vjp_pair.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.opiv_variance_pair.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp]
     for name in EXPORTS:
         if name not in ("opiv_last_error", "opiv_version", "opiv_ctx_stream"):
             getattr(lib, name).restype = ctypes.c_int
@@ -300,6 +303,27 @@ class Plan:
     def run_async(self, a_ptr: int, b_ptr: int, out_ptr: int) -> None:
         """Enqueue on the context stream; device pointers only."""
         _check(self._lib.opiv_run_pair_async(self.handle, a_ptr, b_ptr, out_ptr, self.count))
+
+    def vjp(self, a, b, cot, fwd=None):
+        """Frame gradients of sum_i cot[i] . (u_i, v_i): (grad_a, grad_b) float32 (h, pitch)."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        b = np.ascontiguousarray(b, dtype=self.dtype)
+        cot = np.ascontiguousarray(cot, dtype=np.float64).reshape(self.count, 2)
+        pitch = self.desc.pitch or self.desc.img_w
+        ga = np.zeros((self.desc.img_h, pitch), np.float32)
+        gb = np.zeros((self.desc.img_h, pitch), np.float32)
+        _check(self._lib.opiv_vjp_pair(self.handle, _ptr(a), _ptr(b), None if fwd is None else _ptr(fwd),
+                                       cot.ctypes.data, ga.ctypes.data, gb.ctypes.data))
+        return ga, gb
+
+    def variance(self, a, b, sigma: float, fwd=None) -> np.ndarray:
+        """(n, 2) variances of (u, v) under i.i.d. pixel noise sigma."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        b = np.ascontiguousarray(b, dtype=self.dtype)
+        var = np.zeros((self.count, 2), np.float64)
+        _check(self._lib.opiv_variance_pair(self.handle, _ptr(a), _ptr(b), None if fwd is None else _ptr(fwd),
+                                            sigma, var.ctypes.data))
+        return var
 
 
 class Batch:

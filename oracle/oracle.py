@@ -78,6 +78,14 @@ def load():
                                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         lib.oref_process_pair.argtypes = [ctypes.POINTER(Job), vp]
         lib.oref_process_pair.restype = ctypes.c_long
+        lib.oref_window_jacobian.argtypes = [vp, vp, vp, u32, ctypes.c_int, vp, vp, vp, vp]
+        lib.oref_window_jacobian.restype = ctypes.c_int
+        lib.oref_vjp_pair.argtypes = [ctypes.POINTER(Job), vp, vp, vp]
+        lib.oref_vjp_pair.restype = ctypes.c_long
+        lib.oref_variance_pair.argtypes = [ctypes.POINTER(Job), ctypes.c_double, vp]
+        lib.oref_variance_pair.restype = ctypes.c_long
+        lib.oref_process_window.argtypes = [vp, vp, vp, u32, ctypes.c_int32, ctypes.c_int32, u32, ctypes.c_int,
+                                            ctypes.c_int, vp, vp]
         _lib = lib
     return _lib
 
@@ -205,3 +213,57 @@ def have_reference_harness() -> bool:
 def run_reference(args, **kw):
     return subprocess.run([str(REF_BIN)] + [str(x) for x in args], check=True,
                           capture_output=True, text=True, **kw)
+
+
+# ---------------------------------------------------------------- adjoint (FP64)
+
+def window_vector(wa, wb, limit_search=False):
+    """(u, v, flags) of one window at origin 0 (main.cpp:189-237 in FP64)."""
+    wa = np.ascontiguousarray(wa, dtype=np.float64)
+    wb = np.ascontiguousarray(wb, dtype=np.float64)
+    w = wa.shape[0]
+    f = FFT(w)
+    plane = np.empty_like(wa)
+    out = np.zeros(1, dtype=VECTOR_DTYPE)
+    load().oref_process_window(f.handle, wa.ctypes.data, wb.ctypes.data, w, 0, 0, w, 0,
+                               1 if limit_search else 0, plane.ctypes.data, out.ctypes.data)
+    return out[0]
+
+
+def window_jacobian(wa, wb, limit_search=False):
+    """(ju_a, ju_b, jv_a, jv_b) FP64 W x W arrays, or None without a valid fit."""
+    wa = np.ascontiguousarray(wa, dtype=np.float64)
+    wb = np.ascontiguousarray(wb, dtype=np.float64)
+    w = wa.shape[0]
+    f = FFT(w)
+    j = np.zeros((4, w, w))
+    rc = load().oref_window_jacobian(f.handle, wa.ctypes.data, wb.ctypes.data, w, 1 if limit_search else 0,
+                                     j[0].ctypes.data, j[1].ctypes.data, j[2].ctypes.data, j[3].ctypes.data)
+    return None if rc else j
+
+
+def _job(a, b, win, off_x, off_y, limit_search):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    h, w = a.shape
+    return a, b, Job(a.ctypes.data, b.ctypes.data, _DT[a.dtype], w, h, w, win, off_x, off_y, 0,
+                     1 if limit_search else 0, 1, 0, 0)
+
+
+def vjp_pair(a, b, win, off_x, off_y, cot, limit_search=False):
+    a, b, job = _job(a, b, win, off_x, off_y, limit_search)
+    cot = np.ascontiguousarray(cot, dtype=np.float64)
+    ga = np.zeros(a.shape)
+    gb = np.zeros(a.shape)
+    if load().oref_vjp_pair(ctypes.byref(job), cot.ctypes.data, ga.ctypes.data, gb.ctypes.data) < 0:
+        raise ValueError("oref_vjp_pair failed")
+    return ga, gb
+
+
+def variance_pair(a, b, win, off_x, off_y, sigma, limit_search=False):
+    a, b, job = _job(a, b, win, off_x, off_y, limit_search)
+    _, xy = grid(a.shape[1], a.shape[0], win, win, off_x, off_y)
+    var = np.zeros((len(xy), 2))
+    if load().oref_variance_pair(ctypes.byref(job), sigma, var.ctypes.data) < 0:
+        raise ValueError("oref_variance_pair failed")
+    return var

@@ -570,3 +570,143 @@ long oref_process_pair(const oref_job* job, oref_vector* out)
     oref_fft_destroy(f);
     return (long)(end - first);
 }
+
+/* ------------------------------------------------------------------ */
+/* adjoint of the per-window vector (no reference counterpart)        */
+/* ------------------------------------------------------------------ */
+
+/* d(num/den)/d(l, c, r) for num = ln l - ln r, den = 2 (ln l + ln r - 2 ln c) */
+static void gauss3_grad(double l, double c, double r, double* gl, double* gc, double* gr)
+{
+    const double L = log(l), R = log(r), Cc = log(c);
+    const double num = L - R, den = 2.0 * (L + R - 2.0 * Cc);
+    if (den == 0.0) { *gl = *gc = *gr = 0.0; return; }
+    const double d2 = den * den;
+    *gl = (den - 2.0 * num) / d2 / l;
+    *gr = (-den - 2.0 * num) / d2 / r;
+    *gc = 4.0 * num / d2 / c;
+}
+
+int oref_window_jacobian(const oref_fft* f, const double* wa, const double* wb, uint32_t win,
+                         int limit_search, double* ju_a, double* ju_b, double* jv_a, double* jv_b)
+{
+    const uint32_t W = win;
+    const size_t n = (size_t)W * W;
+    memset(ju_a, 0, n * sizeof(double));
+    memset(ju_b, 0, n * sizeof(double));
+    memset(jv_a, 0, n * sizeof(double));
+    memset(jv_b, 0, n * sizeof(double));
+    double* plane = (double*)malloc(sizeof(double) * n);
+    oref_cross_correlate(f, wa, wb, plane);
+    uint32_t rx = 0, ry = 0, rw = W, rh = W;
+    if (limit_search) {
+        int32_t dx, dy;
+        oref_dilate(W, W, 0.5, &dx, &dy, &rw, &rh);
+        rx = (uint32_t)dx;
+        ry = (uint32_t)dy;
+    }
+    oref_peak peaks[2];
+    const size_t np = oref_find_peaks(plane, W, W, rx, ry, rw, rh, 2, 1, peaks, NULL);
+    if (np != 2) { free(plane); return -1; }
+    const int32_t px = peaks[0].x, py = peaks[0].y;
+    const double l = plane[(size_t)py * W + px - 1], c = plane[(size_t)py * W + px],
+                 r = plane[(size_t)py * W + px + 1], t = plane[(size_t)(py - 1) * W + px],
+                 bt = plane[(size_t)(py + 1) * W + px];
+    free(plane);
+    double fx, fy;
+    {
+        /* the forward fit must be finite for the derivative to exist */
+        const double dx = (log(l) - log(r)), dxd = 2.0 * (log(l) + log(r) - 2.0 * log(c));
+        const double dy = (log(t) - log(bt)), dyd = 2.0 * (log(t) + log(bt) - 2.0 * log(c));
+        fx = dxd == 0.0 ? 0.0 : dx / dxd;
+        fy = dyd == 0.0 ? 0.0 : dy / dyd;
+        if (!isfinite(fx) || !isfinite(fy)) return -1;
+    }
+    /* u = W/2 - px - dx: du/dP = -d(dx)/dP; the plane holds W*W times the
+     * correlation, a scale that cancels in the logarithmic ratios */
+    double gl, gc_x, gr, gt, gc_y, gb;
+    gauss3_grad(l, c, r, &gl, &gc_x, &gr);
+    gauss3_grad(t, c, bt, &gt, &gc_y, &gb);
+    const double scale = (double)n;
+    /* shifts: swapped plane (h, w) holds C[(h + W/2) % W][(w + W/2) % W] */
+    const uint32_t sy = ((uint32_t)py + W / 2) % W, sx = ((uint32_t)px + W / 2) % W;
+    struct { uint32_t sy, sx; double gu, gv; } tap[5] = {
+        { sy, (sx + W - 1) % W, -gl * scale, 0.0 },
+        { sy, sx, -gc_x * scale, -gc_y * scale },
+        { sy, (sx + 1) % W, -gr * scale, 0.0 },
+        { (sy + W - 1) % W, sx, 0.0, -gt * scale },
+        { (sy + 1) % W, sx, 0.0, -gb * scale },
+    };
+    for (int k = 0; k < 5; ++k) {
+        for (uint32_t y = 0; y < W; ++y) {
+            for (uint32_t x = 0; x < W; ++x) {
+                const double bv = wb[(size_t)((y + tap[k].sy) % W) * W + (x + tap[k].sx) % W];
+                const double av = wa[(size_t)((y + W - tap[k].sy) % W) * W + (x + W - tap[k].sx) % W];
+                ju_a[(size_t)y * W + x] += tap[k].gu * bv;
+                jv_a[(size_t)y * W + x] += tap[k].gv * bv;
+                ju_b[(size_t)y * W + x] += tap[k].gu * av;
+                jv_b[(size_t)y * W + x] += tap[k].gv * av;
+            }
+        }
+    }
+    return 0;
+}
+
+static long adjoint_pair(const oref_job* job, const double* cot, double* grad_a, double* grad_b,
+                         double sigma, double* var)
+{
+    uint32_t nx, ny;
+    int rc = oref_grid(job->img_w, job->img_h, job->win, job->win, job->off_x, job->off_y, &nx, &ny, NULL);
+    if (rc) return rc;
+    oref_fft* f = oref_fft_create(job->win, job->win);
+    if (!f) return -10;
+    const size_t total = (size_t)nx * ny;
+    const uint32_t W = job->win;
+    const size_t n = (size_t)W * W;
+    int32_t* xy = (int32_t*)malloc(sizeof(int32_t) * 2 * total);
+    oref_grid(job->img_w, job->img_h, W, W, job->off_x, job->off_y, &nx, &ny, xy);
+    double* wa = (double*)malloc(sizeof(double) * n);
+    double* wb = (double*)malloc(sizeof(double) * n);
+    double* j = (double*)malloc(sizeof(double) * 4 * n);
+    if (grad_a) memset(grad_a, 0, sizeof(double) * job->img_w * job->img_h);
+    if (grad_b) memset(grad_b, 0, sizeof(double) * job->img_w * job->img_h);
+    for (size_t i = 0; i < total; ++i) {
+        const int32_t x0 = xy[2 * i], y0 = xy[2 * i + 1];
+        extract_window(job->a, job->dtype, job->pitch, x0, y0, W, wa);
+        extract_window(job->b, job->dtype, job->pitch, x0, y0, W, wb);
+        const int ok = oref_window_jacobian(f, wa, wb, W, job->limit_search, j, j + n, j + 2 * n, j + 3 * n) == 0;
+        if (var) {
+            double su = 0.0, sv = 0.0;
+            if (ok)
+                for (size_t k = 0; k < n; ++k) {
+                    su += j[k] * j[k] + j[n + k] * j[n + k];
+                    sv += j[2 * n + k] * j[2 * n + k] + j[3 * n + k] * j[3 * n + k];
+                }
+            var[2 * i] = sigma * sigma * su;
+            var[2 * i + 1] = sigma * sigma * sv;
+        }
+        if (grad_a && ok) {
+            const double cu = cot[2 * i], cv = cot[2 * i + 1];
+            for (uint32_t y = 0; y < W; ++y)
+                for (uint32_t x = 0; x < W; ++x) {
+                    const size_t g = (size_t)(y0 + (int32_t)y) * job->img_w + (size_t)(x0 + (int32_t)x);
+                    const size_t k = (size_t)y * W + x;
+                    grad_a[g] += cu * j[k] + cv * j[2 * n + k];
+                    grad_b[g] += cu * j[n + k] + cv * j[3 * n + k];
+                }
+        }
+    }
+    free(j); free(wb); free(wa); free(xy);
+    oref_fft_destroy(f);
+    return (long)total;
+}
+
+long oref_vjp_pair(const oref_job* job, const double* cot, double* grad_a, double* grad_b)
+{
+    return adjoint_pair(job, cot, grad_a, grad_b, 0.0, NULL);
+}
+
+long oref_variance_pair(const oref_job* job, double sigma, double* var)
+{
+    return adjoint_pair(job, NULL, NULL, NULL, sigma, var);
+}

@@ -137,6 +137,29 @@ void oref_process_window(const oref_fft* f, const double* wa, const double* wb, 
                          int32_t blx, int32_t bly, uint32_t img_h, int variant,
                          int limit_search, double* plane_scratch, oref_vector* res);
 
+/* ---- adjoint (no reference counterpart; pinned by FP64 central finite
+ * differences of oref_process_window) ----
+ *
+ * Jacobian of one window's vector (u, v) (main.cpp:227) with respect to the
+ * window's pixels of both frames, in FP64: ju_a[y*W+x] = du/da[y][x], etc.
+ * u = W/2 - px - dx with dx = (ln l - ln r) / (2 (ln l + ln r - 2 ln c))
+ * (image_utils.impl.h:86-94), and each stencil value is the circular
+ * correlation C[s] = sum_n a[n] b[n+s] at a fixed shift s, so
+ * dC[s]/da[n] = b[n+s] and dC[s]/db[m] = a[m-s].  The integer peak is held
+ * fixed (the Jacobian of the argmax is zero almost everywhere).  Returns 0,
+ * or -1 when the window has fewer than two peaks or a non-finite fit. */
+int oref_window_jacobian(const oref_fft* f, const double* wa, const double* wb, uint32_t win,
+                         int limit_search, double* ju_a, double* ju_b, double* jv_a, double* jv_b);
+
+/* sum over windows of cot[2i] * d(u_i) + cot[2i+1] * d(v_i), accumulated
+ * into frame-sized gradients (FP64, row-major, pitch = img_w) */
+long oref_vjp_pair(const oref_job* job, const double* cot, double* grad_a, double* grad_b);
+
+/* per-window variance of (u, v) under i.i.d. pixel noise sigma on both
+ * frames: var[2i] = sigma^2 * sum(ju_a^2 + ju_b^2), var[2i+1] likewise
+ * (0 for windows without a valid fit) */
+long oref_variance_pair(const oref_job* job, double sigma, double* var);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,138 @@
+"""GPU adjoint (VJP of each window's (u, v) with respect to both frames, and
+the propagated noise variance) against the FP64 CPU adjoint of the oracle and
+against FP64 central finite differences (step 1e-3) of the oracle's forward
+path.  Bar: gradients within 1e-4 of the window's max |gradient|, variances
+within 1e-4 relative."""
+import numpy as np
+import pytest
+
+import openpiv_amd as oa
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+GRAD_TOL = 1e-4
+H = 1e-3
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def frames(w=1024, h=1024, seed=5, dtype=np.float32):
+    # the differentiation workload: uniform shift (1.6, -0.8)
+    return oa.synth_pair(w, h, seed=seed, params=(1.6, -0.8), dtype=dtype)
+
+
+def isolated_cotangents(nx, ny, px, py, cu, cv):
+    """cotangent only on windows (ix, iy) with ix % 2 == px, iy % 2 == py:
+    at 50% overlap those windows do not overlap, so the frame gradient inside
+    each is that window's own gradient"""
+    cot = np.zeros((ny, nx, 2))
+    cot[py::2, px::2, 0] = cu
+    cot[py::2, px::2, 1] = cv
+    return cot.reshape(-1, 2)
+
+
+@pytest.mark.parametrize("win", [16, 32, 64])
+def test_vjp_matches_fp64_adjoint(gpu_ctx, win):
+    say(f"vjp vs oracle adjoint win={win}")
+    a, b = frames(512, 512)
+    step = win // 2
+    plan = oa.Plan(gpu_ctx, 512, 512, win, step=(step, step), dtype=np.float32)
+    fwd = plan.run(a, b)
+    xy = plan.grid()
+    nx = 1 + (512 - win) // step
+    ny = len(xy) // nx
+    checked = 0
+    for cu, cv in ((1.0, 0.0), (0.0, 1.0)):
+        cot = isolated_cotangents(nx, ny, 1, 1, cu, cv)
+        ga, gb = plan.vjp(a, b, cot, fwd)
+        for i in np.flatnonzero(cot[:, 0] + cot[:, 1]):
+            x0, y0 = xy[i]
+            j = orc.window_jacobian(a[y0:y0 + win, x0:x0 + win], b[y0:y0 + win, x0:x0 + win])
+            if j is None or fwd["flags"][i] & (oa.FLAG_NEAR_TIE | oa.FLAG_EXACT_TIE):
+                continue
+            ref_a, ref_b = (j[0], j[1]) if cu else (j[2], j[3])
+            got_a = ga[y0:y0 + win, x0:x0 + win]
+            got_b = gb[y0:y0 + win, x0:x0 + win]
+            scale = max(np.abs(ref_a).max(), np.abs(ref_b).max())
+            assert np.abs(got_a - ref_a).max() <= GRAD_TOL * scale, (i, np.abs(got_a - ref_a).max() / scale)
+            assert np.abs(got_b - ref_b).max() <= GRAD_TOL * scale, (i, np.abs(got_b - ref_b).max() / scale)
+            checked += 1
+    assert checked > 20
+
+
+def test_vjp_random_cotangent_whole_frame(gpu_ctx):
+    a, b = frames(256, 256, seed=8)
+    plan = oa.Plan(gpu_ctx, 256, 256, 32, dtype=np.float32)
+    rng = np.random.default_rng(0)
+    cot = rng.normal(size=(plan.count, 2))
+    fwd = plan.run(a, b)
+    # windows the engine flags as near ties contribute with whichever peak
+    # FP32 picked; zero their cotangent so both sides differentiate the same peak
+    cot[(fwd["flags"] & (oa.FLAG_NEAR_TIE | oa.FLAG_EXACT_TIE)) != 0] = 0
+    ga, gb = plan.vjp(a, b, cot)                   # forward computed inside
+    ra, rb = orc.vjp_pair(a, b, 32, 16, 16, cot)
+    for got, ref in ((ga, ra), (gb, rb)):
+        assert np.abs(got - ref).max() <= GRAD_TOL * np.abs(ref).max()
+
+
+def test_variance_matches_fp64(gpu_ctx):
+    a, b = frames(512, 512, seed=11)
+    plan = oa.Plan(gpu_ctx, 512, 512, 32, dtype=np.float32)
+    var = plan.variance(a, b, 2.0)
+    ref = orc.variance_pair(a, b, 32, 16, 16, 2.0)
+    fwd = plan.run(a, b)
+    ok = (fwd["flags"] & (oa.FLAG_NEAR_TIE | oa.FLAG_EXACT_TIE | oa.FLAG_FEWER_THAN_2_PEAKS)) == 0
+    assert np.allclose(var[ok], ref[ok], rtol=GRAD_TOL, atol=0)
+    say(f"variance: median std u {np.sqrt(np.median(var[ok, 0])):.4f} px, v {np.sqrt(np.median(var[ok, 1])):.4f} px")
+
+
+def test_vjp_against_central_differences(gpu_ctx):
+    """64 seeded windows of the 1024^2 differentiation pair: GPU gradients
+    against (u(+h) - u(-h)) / 2h of the FP64 oracle, h = 1e-3."""
+    a, b = frames(1024, 1024, seed=5)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    plan = oa.Plan(gpu_ctx, 1024, 1024, 32, dtype=np.float32)
+    fwd = plan.run(a, b)
+    xy = plan.grid()
+    nx = 63
+    rng = np.random.default_rng(64)
+    picks = rng.choice(np.flatnonzero((np.arange(len(xy)) % nx) % 2 == 0), 64, replace=False)
+    # isolate: only even columns of windows carry cotangent, rows every other too
+    picks = [i for i in picks if (i // nx) % 2 == 0][:64]
+    checked = 0
+    for comp in (0, 1):
+        cot = np.zeros((len(xy), 2))
+        cot[picks, comp] = 1.0
+        ga, gb = plan.vjp(a, b, cot, fwd)
+        for i in picks:
+            if fwd["flags"][i] & (oa.FLAG_NEAR_TIE | oa.FLAG_EXACT_TIE | oa.FLAG_FEWER_THAN_2_PEAKS):
+                continue
+            x0, y0 = xy[i]
+            wa, wb = a64[y0:y0 + 32, x0:x0 + 32], b64[y0:y0 + 32, x0:x0 + 32]
+            base = orc.window_vector(wa, wb)
+            for frame, grad in ((0, ga), (1, gb)):
+                y, x = rng.integers(0, 32, 2)
+                plus, minus = [wa.copy(), wb.copy()], [wa.copy(), wb.copy()]
+                plus[frame][y, x] += H
+                minus[frame][y, x] -= H
+                vp, vm = orc.window_vector(*plus), orc.window_vector(*minus)
+                if (vp["px"], vp["py"], vm["px"], vm["py"]) != (base["px"], base["py"]) * 2:
+                    continue
+                fd = (vp["uv"[comp]] - vm["uv"[comp]]) / (2 * H)
+                g = grad[y0 + y, x0 + x]
+                scale = np.abs(grad[y0:y0 + 32, x0:x0 + 32]).max()
+                assert abs(g - fd) <= GRAD_TOL * scale, (i, frame, comp, g, fd, scale)
+                checked += 1
+    say(f"finite-difference checks: {checked}")
+    assert checked >= 100
+
+
+def test_adjoint_rejects_real_variant(gpu_ctx):
+    plan = oa.Plan(gpu_ctx, 128, 128, 32, variant=oa.REAL_COMPAT)
+    z = np.zeros((128, 128), np.float32)
+    with pytest.raises(oa.OpivError) as e:
+        plan.variance(z, z, 1.0)
+    assert e.value.code == 6
