@@ -70,12 +70,21 @@ class HipEvents:
         return ms.value
 
 
-def workload(win: int):
+def workload(win: int, dtype: str = "f32"):
+    """4096^2 frame pair of the benchmark workloads: f32 with solid-body
+    rotation (|d| = 6 px at the farthest window), or uint16 (I0 ~ U(1000,
+    60000), noise 50, shift (0.4, 1.1)), or uint8 (I0 ~ U(30, 250), noise 3,
+    same shift)."""
     import openpiv_amd as oa
     w = 4096
     step = win // 2
-    r = float(np.hypot(w / 2 - win / 2, w / 2 - win / 2))
-    a, b = oa.synth_pair(w, w, seed=2, field=oa.FIELD_ROTATION, params=(6.0, r), dtype=np.float32)
+    if dtype == "f32":
+        r = float(np.hypot(w / 2 - win / 2, w / 2 - win / 2))
+        a, b = oa.synth_pair(w, w, seed=2, field=oa.FIELD_ROTATION, params=(6.0, r), dtype=np.float32)
+    elif dtype == "u16":
+        a, b = oa.synth_pair(w, w, seed=4, i0=(1000.0, 60000.0), noise=50.0, params=(0.4, 1.1), dtype=np.uint16)
+    else:
+        a, b = oa.synth_pair(w, w, seed=4, i0=(30.0, 250.0), noise=3.0, params=(0.4, 1.1), dtype=np.uint8)
     return a, b, w, step
 
 
@@ -259,6 +268,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--win", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--path", choices=["auto", "fft", "direct"], default="auto",
+                    help="correlation path: fused FP32 FFT, BF16-split MFMA direct (16^2), or the faster")
+    ap.add_argument("--dtype", choices=["f32", "u16", "u8"], default="f32")
     ap.add_argument("--adjoint", action="store_true",
                     help="1024^2 f32 differentiation workload: forward vs VJP vs variance (prints its own line)")
     ap.add_argument("--batch", action="store_true",
@@ -284,47 +296,56 @@ def main():
         return
 
     win = args.win
-    log(f"[bench] generating 4096x4096 f32 pair, win={win}")
-    a, b, w, step = workload(win)
+    log(f"[bench] generating 4096x4096 {args.dtype} pair, win={win}")
+    a, b, w, step = workload(win, args.dtype)
     ctx = oa.Context(0)
-    plan = oa.Plan(ctx, w, w, win, step=(step, step), dtype=np.float32)
-    n = plan.count
+    np_dtype = {"f32": np.float32, "u16": np.uint16, "u8": np.uint8}[args.dtype]
+    variants = {"auto": oa.AUTO, "fft": oa.CIRCULAR_FFT_FP32, "direct": oa.DIRECT_BF16_SPLIT}
     # frames resident in HBM before timing (device buffers from the engine's
     # own HIP runtime)
     da, db = oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)
-    dout = oa.DeviceBuffer(n * RESULT_BYTES)
-
     ev = HipEvents()
-    stream = ctx.stream
-    log(f"[bench] warmup {args.warmup}")
-    for _ in range(args.warmup):
-        plan.run_async(da.ptr, db.ptr, dout.ptr)
-    ctx.synchronize()
-
-    log(f"[bench] timing {args.steps} steps")
-    e0, e1 = ev.create(), ev.create()
     hip = oa._hip_runtime()
-    hip.hipDeviceSynchronize()
-    t0 = time.perf_counter()
-    ev.record(e0, stream)
-    for _ in range(args.steps):
-        plan.run_async(da.ptr, db.ptr, dout.ptr)
-    ev.record(e1, stream)
-    ctx.synchronize()
-    hip.hipDeviceSynchronize()
-    wall = time.perf_counter() - t0
-    ms = ev.elapsed_ms(e0, e1)
-    ms_per_step = ms / args.steps
+
+    def measure(variant):
+        plan = oa.Plan(ctx, w, w, win, step=(step, step), dtype=np_dtype, variant=variant)
+        n = plan.count
+        dout = oa.DeviceBuffer(n * RESULT_BYTES)
+        for _ in range(args.warmup):
+            plan.run_async(da.ptr, db.ptr, dout.ptr)
+        ctx.synchronize()
+        e0, e1 = ev.create(), ev.create()
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        ev.record(e0, ctx.stream)
+        for _ in range(args.steps):
+            plan.run_async(da.ptr, db.ptr, dout.ptr)
+        ev.record(e1, ctx.stream)
+        ctx.synchronize()
+        hip.hipDeviceSynchronize()
+        wall = time.perf_counter() - t0
+        ms = ev.elapsed_ms(e0, e1) / args.steps
+        res = dout.download(np.zeros(n, oa.RESULT_DTYPE))
+        return n, ms, wall, res
+
+    log(f"[bench] timing {args.steps} steps (plain FFT path)")
+    n, plain_ms, wall, res = measure(oa.CIRCULAR_FFT_FP32)
+    chosen = args.path
+    if args.path == "auto":
+        chosen = "direct" if oa.direct_chosen(win) else "fft"
+    ms_per_step = plain_ms
+    if chosen == "direct":
+        log(f"[bench] timing {args.steps} steps (direct MFMA path)")
+        n, ms_per_step, wall, res = measure(oa.DIRECT_BF16_SPLIT)
     value = n / (ms_per_step * 1e-3)
 
-    res = dout.download(np.zeros(n, oa.RESULT_DTYPE))
     log("[bench] accuracy sample vs FP64 oracle")
     acc = accuracy(res, a, b, win, step)
 
-    frame_bytes = 2 * w * w * 4
+    frame_bytes = 2 * w * w * np.dtype(np_dtype).itemsize
     alg_bytes = frame_bytes + n * RESULT_BYTES
     achieved = alg_bytes / (ms_per_step * 1e-3) / 1e9
-    key = f"fft_fp32_w{win}_4096_f32"
+    key = f"{'direct_bf16' if chosen == 'direct' else 'fft_fp32'}_w{win}_4096_{args.dtype}"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(key),
                 "algorithmic_bytes_per_launch": alg_bytes,
@@ -336,15 +357,18 @@ def main():
         log(f"[bench] cpu baseline, {threads} threads")
         cpu = cpu_baseline(a, b, win, step, threads)
 
+    path_name = {"fft": "fft_fp32_fused", "direct": "direct_bf16_split_mfma"}[chosen]
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "vectors/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"4096x4096 f32 frame pair, rotation |d|max 6 px, {win}x{win} windows "
-                               f"step {step} ({n} windows), fused FP32 FFT correlation + find_peaks + "
-                               f"Gaussian fit", "windows": n, "window": win, "step": step,
-                   "path": "fft_fp32_fused", "frames_resident_in_hbm": True},
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.dtype, "data": "synthetic",
+        "config": {"workload": f"4096x4096 {args.dtype} frame pair, {win}x{win} windows step {step} ({n} windows), "
+                               f"correlation + find_peaks + Gaussian fit fused, chosen path {path_name}",
+                   "windows": n, "window": win, "step": step, "path": path_name,
+                   "frames_resident_in_hbm": True},
+        "plain_gpu": {"path": "fft_fp32_fused", "ms_per_step": round(plain_ms, 5),
+                      "value": round(n / (plain_ms * 1e-3), 1)},
         "roofline": roofline, "cpu_baseline": cpu, "accuracy": acc,
         "wall_s_timed_region": round(wall, 4),
     }

@@ -50,7 +50,14 @@ typedef enum opiv_variant {
     /* algos::FFT::cross_correlate_real (fft.h:208-227) including its
      * unpack (fft.h:166-183) that leaves spectral row 0, row H/2 and
      * column 0 zero */
-    OPIV_REAL_COMPAT = 1
+    OPIV_REAL_COMPAT = 1,
+    /* the exact circular correlation computed directly (no FFT) as an
+     * im2col GEMM on BF16 MFMA with split operands and FP32 accumulation:
+     * u8 exact, u16 exact products, f32/f64 three-term split; 16x16 windows */
+    OPIV_DIRECT_BF16_SPLIT = 2,
+    /* the faster of OPIV_CIRCULAR_FFT_FP32 and OPIV_DIRECT_BF16_SPLIT for
+     * the plan's window size and pixel type (same semantics) */
+    OPIV_AUTO = 3
 } opiv_variant;
 
 typedef enum opiv_flag {

@@ -11,6 +11,7 @@
 #include <string>
 
 #include "adjoint.hpp"
+#include "corr_direct.hpp"
 #include "corr_fft.hpp"
 #include "ctx.hpp"
 #include "openpiv_amd.h"
@@ -112,6 +113,15 @@ int opiv_grid(uint32_t img_w, uint32_t img_h, uint32_t win_w, uint32_t win_h,
     return OPIV_OK;
 }
 
+// OPIV_AUTO: the measured faster path per window size (both compute the
+// same circular correlation): the MFMA direct path for 16 x 16 windows, the
+// fused FFT otherwise
+static int32_t auto_variant(int win, int dtype)
+{
+    (void)dtype;
+    return opiv::direct_supported(win) ? OPIV_DIRECT_BF16_SPLIT : OPIV_CIRCULAR_FFT_FP32;
+}
+
 int opiv_plan_create(opiv_ctx* ctx, const opiv_plan_desc* desc, opiv_plan** out)
 {
     if (!ctx || !desc || !out) return fail(OPIV_E_INVALID, "opiv_plan_create: null argument");
@@ -122,8 +132,10 @@ int opiv_plan_create(opiv_ctx* ctx, const opiv_plan_desc* desc, opiv_plan** out)
     if (win < 8 || win > 64)
         return fail(OPIV_E_UNSUPPORTED, "window %u outside the supported range [8, 64]", win);
     if (dtype_size(desc->dtype) == 0) return fail(OPIV_E_INVALID, "unknown dtype %d", desc->dtype);
-    if (desc->variant != OPIV_CIRCULAR_FFT_FP32 && desc->variant != OPIV_REAL_COMPAT)
+    if (desc->variant < OPIV_CIRCULAR_FFT_FP32 || desc->variant > OPIV_AUTO)
         return fail(OPIV_E_UNSUPPORTED, "unknown variant %d", desc->variant);
+    if (desc->variant == OPIV_DIRECT_BF16_SPLIT && !opiv::direct_supported((int)win))
+        return fail(OPIV_E_UNSUPPORTED, "the direct BF16 path supports 16x16 windows, not %u", win);
     const size_t pitch = desc->pitch ? desc->pitch : desc->img_w;
     if (pitch < desc->img_w) return fail(OPIV_E_INVALID, "pitch %zu smaller than width %u", pitch, desc->img_w);
 
@@ -131,6 +143,7 @@ int opiv_plan_create(opiv_ctx* ctx, const opiv_plan_desc* desc, opiv_plan** out)
     p->ctx = ctx;
     p->desc = *desc;
     p->desc.pitch = pitch;
+    if (desc->variant == OPIV_AUTO) p->desc.variant = auto_variant((int)win, desc->dtype);
     int rc = OPIV_OK;
     if (desc->step_x && desc->step_y) {
         p->off_x = desc->step_x;
@@ -197,6 +210,14 @@ static opiv::CorrParams plan_params(const opiv_plan* plan, const void* a, const 
     return p;
 }
 
+// the plan's forward kernel
+static hipError_t launch_plan(const opiv_plan* plan, const opiv::CorrParams& p, hipStream_t stream)
+{
+    if (plan->desc.variant == OPIV_DIRECT_BF16_SPLIT)
+        return opiv::launch_corr_direct(p, (int)plan->desc.win, plan->desc.dtype, false, stream);
+    return opiv::launch_corr_fft(p, (int)plan->desc.win, plan->desc.dtype, false, stream);
+}
+
 int opiv_run_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, opiv_window_result* d_out, size_t n)
 {
     if (!plan || !d_a || !d_b || !d_out) return fail(OPIV_E_INVALID, "opiv_run_pair_async: null argument");
@@ -204,7 +225,7 @@ int opiv_run_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, opiv_
     if (n != count) return fail(OPIV_E_SIZE, "result count %zu differs from the grid's %zu windows", n, count);
     HIP_TRY(hipSetDevice(plan->ctx->device));
     opiv::CorrParams p = plan_params(plan, d_a, d_b, d_out);
-    HIP_TRY(opiv::launch_corr_fft(p, (int)plan->desc.win, plan->desc.dtype, false, plan->ctx->stream));
+    HIP_TRY(launch_plan(plan, p, plan->ctx->stream));
     return OPIV_OK;
 }
 
@@ -235,7 +256,7 @@ int opiv_run_pair(opiv_plan* plan, const void* a, const void* b, opiv_window_res
         dout = (opiv_window_result*)ctx->stage_out.p;
     }
     opiv::CorrParams p = plan_params(plan, da, db, dout);
-    HIP_TRY(opiv::launch_corr_fft(p, (int)plan->desc.win, plan->desc.dtype, false, ctx->stream));
+    HIP_TRY(launch_plan(plan, p, ctx->stream));
     if (!out_dev)
         HIP_TRY(hipMemcpyAsync(out, dout, count * sizeof(opiv_window_result), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -248,8 +269,11 @@ static int correlate_stacked(opiv_ctx* ctx, uint32_t win, size_t n, int variant,
     if (!ctx || !a || !b || !planes) return fail(OPIV_E_INVALID, "correlate: null argument");
     if (!is_pow2(win)) return fail(OPIV_E_NOT_POW2, "dimensions must be power of 2: [%u,%u]", win, win);
     if (win < 8 || win > 64) return fail(OPIV_E_UNSUPPORTED, "window %u outside the supported range [8, 64]", win);
-    if (variant != OPIV_CIRCULAR_FFT_FP32 && variant != OPIV_REAL_COMPAT)
+    if (variant < OPIV_CIRCULAR_FFT_FP32 || variant > OPIV_AUTO)
         return fail(OPIV_E_UNSUPPORTED, "unknown variant %d", variant);
+    if (variant == OPIV_AUTO) variant = auto_variant((int)win, OPIV_F64);
+    if (variant == OPIV_DIRECT_BF16_SPLIT && !opiv::direct_supported((int)win))
+        return fail(OPIV_E_UNSUPPORTED, "the direct BF16 path supports 16x16 windows, not %u", win);
     if (n == 0) return OPIV_OK;
     if (n > 0x7fffffffull / win) return fail(OPIV_E_UNSUPPORTED, "too many windows");
     HIP_TRY(hipSetDevice(ctx->device));
@@ -291,7 +315,10 @@ static int correlate_stacked(opiv_ctx* ctx, uint32_t win, size_t n, int variant,
     p.real_compat = variant == OPIV_REAL_COMPAT;
     p.tie_tol = kTieTol;
     p.planes = dplanes;
-    HIP_TRY(opiv::launch_corr_fft(p, (int)win, OPIV_F64, true, ctx->stream));
+    if (variant == OPIV_DIRECT_BF16_SPLIT)
+        HIP_TRY(opiv::launch_corr_direct(p, (int)win, OPIV_F64, true, ctx->stream));
+    else
+        HIP_TRY(opiv::launch_corr_fft(p, (int)win, OPIV_F64, true, ctx->stream));
     if (!out_dev) HIP_TRY(hipMemcpyAsync(planes, dplanes, bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return OPIV_OK;
@@ -315,8 +342,8 @@ static int adjoint_setup(opiv_plan* plan, const void* a, const void* b, const op
                          opiv::AdjParams& p)
 {
     opiv_ctx* ctx = plan->ctx;
-    if (plan->desc.variant != OPIV_CIRCULAR_FFT_FP32)
-        return fail(OPIV_E_UNSUPPORTED, "the adjoint supports the circular correlation variant only");
+    if (plan->desc.variant == OPIV_REAL_COMPAT)
+        return fail(OPIV_E_UNSUPPORTED, "the adjoint supports the circular correlation only");
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t count = (size_t)plan->nx * plan->ny;
     const size_t frame_bytes = plan->desc.pitch * plan->desc.img_h * dtype_size(plan->desc.dtype);
@@ -330,7 +357,7 @@ static int adjoint_setup(opiv_plan* plan, const void* a, const void* b, const op
     } else {
         HIP_TRY(ctx->stage_fwd.ensure(count * sizeof(opiv_window_result)));
         opiv::CorrParams cp = plan_params(plan, da, db, (opiv_window_result*)ctx->stage_fwd.p);
-        HIP_TRY(opiv::launch_corr_fft(cp, (int)plan->desc.win, plan->desc.dtype, false, ctx->stream));
+        HIP_TRY(launch_plan(plan, cp, ctx->stream));
         dfwd = ctx->stage_fwd.p;
     }
     p = opiv::AdjParams{};

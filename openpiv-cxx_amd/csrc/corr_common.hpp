@@ -1,0 +1,283 @@
+// corr_common.hpp -- device code shared by the correlation kernels: frame
+// loads, window geometry, and the common tail that turns quadrant-swapped
+// correlation planes held as one column per lane into results:
+// find_peaks(.., 2, 1) (image_utils.impl.h:37-74), fit_simple_gaussian
+// (image_utils.impl.h:76-101) and the vector of
+// examples/process/main.cpp:220-234.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "corr_fft.hpp"
+#include "fft_reg.hpp"
+
+namespace opiv {
+
+using fftreg::sfor;
+
+namespace {
+
+// LDS writes of this wave become visible to its later LDS reads; no
+// instruction is moved across.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// value of lane+1 / lane-1 (whole-wave DPP shifts; the group-edge lanes
+// receive a neighbour group's value, which the scan never uses)
+__device__ __forceinline__ float from_right(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_left(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+// frames are global memory: address-space-1 pointers give global (not flat)
+// loads even when the pointer comes from the batch table
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+
+template <int DT>
+__device__ __forceinline__ float load_px(const void* base, size_t idx)
+{
+    if constexpr (DT == 0) return (float)((gptr<uint8_t>)base)[idx];
+    else if constexpr (DT == 1) return (float)((gptr<uint16_t>)base)[idx];
+    else if constexpr (DT == 2) return ((gptr<float>)base)[idx];
+    else return (float)((gptr<double>)base)[idx];
+}
+
+// ordering of find_peaks' sort (descending value); equal values keep scan
+// order, i.e. the smaller row-major index comes first
+__device__ __forceinline__ bool better(float va, int ia, float vb, int ib)
+{
+    return va > vb || (va == vb && ia < ib);
+}
+
+} // namespace
+
+// Where one window lives: frames, pitch, bottom-left corner, frame height,
+// the index of its result and whether it is a real window of the launch
+// (task slots past the end of a pair's range recompute that pair's last
+// window and store nothing).
+struct WinGeom {
+    const void* a;
+    const void* b;
+    size_t pitch;
+    int x0, y0;
+    uint32_t img_h;
+    int out;
+    bool valid;
+};
+
+template <class T>
+using gptr_t = const __attribute__((address_space(1))) T*;
+
+// Geometry of window `local` of a launch: the single pair of the params, or
+// (batch) the pair whose window range [first, first + nx * ny) holds it.
+template <bool BATCH>
+__device__ __forceinline__ WinGeom window_geom(const CorrParams& p, int local)
+{
+    WinGeom g;
+    g.valid = local < p.n_windows;
+    if constexpr (!BATCH) {
+        const int l = g.valid ? local : p.n_windows - 1;
+        const int gid = p.first + l;
+        const int ix = gid % p.nx, iy = gid / p.nx;
+        g.a = p.a;
+        g.b = p.b;
+        g.pitch = p.pitch;
+        g.x0 = p.x_start + ix * p.step_x;
+        g.y0 = p.y_start + iy * p.step_y;
+        g.img_h = p.img_h;
+        g.out = l;
+    } else {
+        const gptr_t<BatchPair> pairs = (gptr_t<BatchPair>)p.pairs;
+        const int l = g.valid ? local : p.n_windows - 1;
+        int lo = 0, hi = p.npairs;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (pairs[mid].first <= l) lo = mid;
+            else hi = mid;
+        }
+        const auto& bp = pairs[lo];
+        const int n = bp.nx * bp.ny;
+        int gid = l - bp.first;
+        if (gid >= n) {
+            g.valid = false;
+            gid = n - 1;
+        }
+        const int ix = gid % bp.nx, iy = gid / bp.nx;
+        g.a = bp.a;
+        g.b = bp.b;
+        g.pitch = bp.pitch;
+        g.x0 = bp.x_start + ix * bp.step_x;
+        g.y0 = bp.y_start + iy * bp.step_y;
+        g.img_h = bp.img_h;
+        g.out = bp.out_first + gid;
+    }
+    return g;
+}
+
+// find_peaks(.., 2, 1) over one quadrant-swapped plane column held by lane
+// t: strict 4-neighbour maxima on rows [hlo, hhi] and columns [wlo, whi]
+// (image_utils.impl.h:48-64), reduced to the largest value (and its scan
+// index h*W + t; equal values keep scan order) and the second largest value
+// over the W lanes of the window.
+template <int W>
+__device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, int hhi, bool col_ok,
+                                          float& v1, int& i1, float& v2)
+{
+    v1 = -INFINITY;
+    v2 = -INFINITY;
+    int h1 = 0x7fff;
+    sfor<W - 3>([&](auto Hm) {
+        constexpr int h = decltype(Hm)::value + 1;
+        const float c = s[h];
+        const float nb = fmaxf(fmaxf(from_left(c), from_right(c)), fmaxf(s[h - 1], s[h + 1]));
+        const bool ismax = col_ok && h >= hlo && h <= hhi && c > nb;
+        const float mc = ismax ? c : -INFINITY;
+        const bool gt = mc > v1;                  // rows arrive in scan order: ties keep the first
+        v2 = fmaxf(v2, fminf(mc, v1));
+        h1 = gt ? h : h1;
+        v1 = gt ? mc : v1;
+    });
+    i1 = h1 * W + t;
+#pragma unroll
+    for (int off = W / 2; off >= 1; off >>= 1) {
+        const float ov1 = __shfl_xor(v1, off, 64);
+        const float ov2 = __shfl_xor(v2, off, 64);
+        const int oi1 = __shfl_xor(i1, off, 64);
+        v2 = fmaxf(fmaxf(v2, ov2), fminf(v1, ov1));
+        const bool mine = better(v1, i1, ov1, oi1);
+        v1 = mine ? v1 : ov1;
+        i1 = mine ? i1 : oi1;
+    }
+}
+
+// Results of the ROUNDS windows of this lane group from their swapped planes
+// s[r][h] (column t of each): find_peaks, the FP64 fit with the 5 * ROUNDS
+// logs spread over the group's lanes, and the vector; or, with PLANES, the
+// planes themselves.
+template <int W, int ROUNDS, bool PLANES>
+__device__ __forceinline__ void finish_windows(const CorrParams& p, const float (&s)[ROUNDS][W],
+                                               const WinGeom (&geo)[ROUNDS], const bool (&active)[ROUNDS],
+                                               int lane, int t)
+{
+    const int gbase = lane - t;
+    if constexpr (PLANES) {
+        sfor<ROUNDS>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            if (active[r]) {
+                double* dst = p.planes + (size_t)geo[r].out * W * W + t;
+                sfor<W>([&](auto Hh) {
+                    constexpr int h = decltype(Hh)::value;
+                    dst[h * W] = (double)s[r][h];
+                });
+            }
+        });
+        return;
+    } else {
+        // search region: whole plane, or the limit-search view rect::dilate(0.5)
+        const int r0 = p.limit_search ? W / 4 : 0;
+        const int rlen = p.limit_search ? W / 2 : W;
+        const int hlo = r0 + 1, hhi = r0 + rlen - 3;
+        const bool col_ok = t >= r0 + 1 && t <= r0 + rlen - 2;
+
+        float v1[ROUNDS], v2[ROUNDS], st[ROUNDS][5];
+        int px[ROUNDS], py[ROUNDS];
+        bool two[ROUNDS];
+        sfor<ROUNDS>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            int i1;
+            scan_top2<W>(s[r], t, hlo, hhi, col_ok, v1[r], i1, v2[r]);
+            two[r] = v2[r] > -INFINITY;
+            px[r] = two[r] ? (i1 & (W - 1)) : 1;
+            py[r] = two[r] ? (i1 / W) : 1;
+            // stencil (l, c, r, up, down) around the peak: the centre column
+            // comes from lane px, the horizontal neighbours from lanes px -+ 1
+            float mid = 0.f, up = 0.f, dn = 0.f;
+            sfor<W - 2>([&](auto Hm) {
+                constexpr int h = decltype(Hm)::value + 1;
+                const bool at = h == py[r];
+                mid = at ? s[r][h] : mid;
+                up = at ? s[r][h - 1] : up;
+                dn = at ? s[r][h + 1] : dn;
+            });
+            st[r][0] = __shfl(mid, gbase + px[r] - 1, 64);
+            st[r][1] = __shfl(mid, gbase + px[r], 64);
+            st[r][2] = __shfl(mid, gbase + px[r] + 1, 64);
+            st[r][3] = __shfl(up, gbase + px[r], 64);
+            st[r][4] = __shfl(dn, gbase + px[r], 64);
+        });
+
+        // ---- FP64 logs of the 5 * ROUNDS stencil values, one per lane ----
+        constexpr int NV = 5 * ROUNDS;
+        constexpr int NP = (NV + W - 1) / W;
+        double lg[NP];
+        sfor<NP>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            float v = 1.f;
+            sfor<NV>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                v = (t + k * W == j) ? st[j / 5][j % 5] : v;
+            });
+            lg[k] = log((double)v);
+        });
+        double ln[NV];
+        sfor<NV>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            ln[j] = __shfl(lg[j / W], gbase + (j % W), 64);
+        });
+
+        sfor<ROUNDS>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            if (!(active[r] && t == r)) return;
+            opiv_window_result res;
+            res.reserved = 0;
+            if (!two[r]) {
+                // main.cpp:214-218: the default (all-zero) record stays
+                res.x = res.y = res.u = res.v = res.sn = 0.0;
+                res.peak0 = res.peak1 = 0.0;
+                res.px = res.py = 0;
+                res.flags = OPIV_FLAG_FEWER_THAN_2_PEAKS;
+            } else {
+                // image_utils.impl.h:86-98 on the logs of (l, c, r) and (up, c, down)
+                const double lnl = ln[5 * r + 0], lnc = ln[5 * r + 1], lnr = ln[5 * r + 2];
+                const double lnu = ln[5 * r + 3], lnd = ln[5 * r + 4];
+                const double numx = lnl - lnr, denx = 2.0 * (lnl + lnr - 2.0 * lnc);
+                const double numy = lnu - lnd, deny = 2.0 * (lnu + lnd - 2.0 * lnc);
+                const double plx = (double)px[r] + (denx == 0.0 ? 0.0 : numx / denx);
+                const double ply = (double)py[r] + (deny == 0.0 ? 0.0 : numy / deny);
+                uint32_t flags = 0;
+                if (!isfinite(plx) || !isfinite(ply)) flags |= OPIV_FLAG_NONFINITE_FIT;
+                const float nmax = fmaxf(fmaxf(st[r][0], st[r][2]), fmaxf(st[r][3], st[r][4]));
+                const float tol = p.tie_tol * fabsf(v1[r]);
+                if (v1[r] - v2[r] <= tol || v1[r] - nmax <= tol) flags |= OPIV_FLAG_NEAR_TIE;
+                if (v1[r] == v2[r]) flags |= OPIV_FLAG_EXACT_TIE;
+                // main.cpp:220-234
+                const int midx = geo[r].x0 + W / 2, midy = geo[r].y0 + W / 2;
+                res.x = (double)midx;
+                res.y = (double)geo[r].img_h - (double)midy;
+                res.u = (double)midx - ((double)geo[r].x0 + plx);
+                res.v = (double)midy - ((double)geo[r].y0 + ply);
+                res.sn = (v2[r] > 0.f) ? (double)v1[r] / (double)v2[r] : 0.0;
+                res.peak0 = (double)v1[r];
+                res.peak1 = (double)v2[r];
+                res.px = px[r];
+                res.py = py[r];
+                res.flags = flags;
+            }
+            p.out[geo[r].out] = res;
+        });
+    }
+}
+
+} // namespace opiv

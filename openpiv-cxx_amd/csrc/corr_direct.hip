@@ -1,0 +1,289 @@
+// corr_direct.hip -- direct circular cross-correlation on BF16 MFMA with
+// split operands (windows of 16 x 16), fused with find_peaks and the
+// Gaussian fit.  Same semantics and outputs as the FFT kernel
+// (corr_fft.hip): algos::FFT::cross_correlate (fft.h:188-206) +
+// swap_quadrants + find_peaks(.., 2, 1) + fit_simple_gaussian + the vector
+// of examples/process/main.cpp:220-234.
+//
+// The correlation of one window is one GEMM.  With k = (y, x),
+//   C[sy][sx] = sum_k P[sy][k] Q[k][sx],  P[sy][(y,x)] = a[(y - sy) % W][x],
+//                                          Q[(y,x)][sx] = b[y][(x + sx) % W]
+// (M = N = 16, K = 256).  Both operands are read from LDS copies of the
+// window: P as aligned 16-byte row pieces of a (the row index carries the
+// shift), Q from doubled rows of b held twice, the second copy shifted by one
+// element, so that every 8-element run starting at any column is
+// dword-aligned in one of the copies.
+//
+// Precision: pixels are split into bf16 terms, x = x0 + x1 (+ x2), and the
+// cross products are accumulated in FP32 by the MFMA:
+//   u8  : x0 only (exact; products and sums of 256 terms stay exact in FP32)
+//   u16 : x0 + x1 exactly (8 + 8 bits), all four products
+//   f32 : three terms (f64 frames are narrowed to f32 first), the six
+//         products x_i y_j with i + j <= 2 (relative error ~2^-24, Henry,
+//         Tang & Heinecke, "Leveraging the bfloat16 AI datatype for
+//         higher-precision computations", ARITH 2019)
+// Products are accumulated smallest first.
+//
+// Mapping: a wave runs 8 windows through the MFMA one after another, then
+// finishes all 8 with the tail shared with the FFT kernel (one FP64 log per
+// lane for 8 windows).  MFMA 16x16x32 lane layout (lane l): A[i][k], B[k][j]
+// with i = j = l % 16 and k = 8 (l / 16) + 0..7; the result D[i][j] has
+// j = l % 16 and i = 4 (l / 16) + r in accumulator r.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <type_traits>
+
+#include "corr_common.hpp"
+#include "corr_direct.hpp"
+#include "fft_reg.hpp"
+
+namespace opiv {
+
+using fftreg::sfor;
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// number of bf16 terms per pixel type
+template <int DT>
+constexpr int nparts() { return DT == 0 ? 1 : (DT == 1 ? 2 : 3); }
+
+__device__ __forceinline__ uint32_t bf16_bits(float x)
+{
+    return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)x);
+}
+__device__ __forceinline__ float bf16_val(uint32_t b)
+{
+    return __builtin_bit_cast(float, b << 16);
+}
+
+// split x into NP bf16 terms (round to nearest each time; every residual is
+// exact in FP32)
+template <int NP>
+__device__ __forceinline__ void split(float x, uint32_t (&t)[NP])
+{
+    float r = x;
+    sfor<NP>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        t[k] = bf16_bits(r);
+        r = r - bf16_val(t[k]);
+    });
+}
+
+// four consecutive pixels of one frame row in their stored type (converted to
+// FP32 only when staged, so that loads stay in flight across earlier windows)
+template <int DT>
+struct RawQuad {
+    using T = std::conditional_t<DT == 0, uint8_t, std::conditional_t<DT == 1, uint16_t,
+              std::conditional_t<DT == 2, float, double>>>;
+    T v[4];
+    __device__ __forceinline__ void load(const void* base, size_t idx)
+    {
+        const gptr<T> q = (gptr<T>)base + idx;
+        sfor<4>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            v[j] = q[j];
+        });
+    }
+    __device__ __forceinline__ float get(int j) const { return (float)v[j]; }
+};
+
+// the value of the next lane of the quad (DPP quad_perm [1, 2, 3, 0])
+__device__ __forceinline__ float from_quad_next(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x39, 0xf, 0xf, false));
+}
+
+} // namespace
+
+// Wave task: 8 windows, (task * 2 + r) * 4 + slot for round r and slot --
+// the numbering and column layout of the 16 x 16 FFT kernel, so the shared
+// tail (finish_windows) turns the planes into results.  Each window in turn:
+// stage its bf16 terms in LDS, run the MFMA GEMM, and park the scaled,
+// quadrant-swapped plane in LDS; then lane (slot, t) reads column t of its
+// two windows' planes.
+template <int DT, bool PLANES>
+__global__ __launch_bounds__(256) void corr_direct16_kernel(CorrParams p)
+{
+    constexpr int W = 16;
+    constexpr int NP = nparts<DT>();
+    constexpr int WAVES = 4;
+    constexpr int ROUNDS = 2, WPW = 4, PER = ROUNDS * WPW;
+    // per wave: the operand staging of one window (rows of a; doubled rows of
+    // b, two copies), later overlaid by the 8 swapped planes
+    typedef uint16_t StageA[NP][W][W];
+    typedef uint16_t StageB[NP][2][W][2 * W];
+    typedef float Planes[PER][W][W + 1];
+    constexpr int STAGE = (int)(sizeof(StageA) + sizeof(StageB));
+    constexpr int BYTES = STAGE > (int)sizeof(Planes) ? STAGE : (int)sizeof(Planes);
+    __shared__ __attribute__((aligned(16))) unsigned char lds[WAVES][BYTES];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int task = blockIdx.x * WAVES + wave;
+    const int ntasks = (p.n_windows + PER - 1) / PER;
+    if (task >= ntasks) return;                 // whole wave idle
+    const int slot = lane / W, t = lane % W;
+
+    bool active[ROUNDS];
+    WinGeom geo[ROUNDS];
+    sfor<ROUNDS>([&](auto R) {
+        constexpr int r = decltype(R)::value;
+        geo[r] = window_geom<false>(p, (task * ROUNDS + r) * WPW + slot);
+        active[r] = geo[r].valid;
+    });
+
+    StageA& lds_a = *reinterpret_cast<StageA*>(&lds[wave][0]);
+    StageB& lds_b = *reinterpret_cast<StageB*>(&lds[wave][sizeof(StageA)]);
+    Planes& lds_p = *reinterpret_cast<Planes*>(&lds[wave][0]);
+
+    const int i16 = lane & 15;                  // MFMA: row of A / column of B / column of D
+    const int g = lane >> 4;                    // MFMA: k group / row quad of D
+    const int srow = lane >> 2, scol = 4 * (lane & 3);   // staging: row and first column
+    floatx4 accs[PER];
+
+    // raw pixels of the next windows, loaded DEPTH windows ahead of their staging
+    constexpr int DEPTH = DT <= 1 ? PER : (DT == 2 ? 4 : 2);
+    RawQuad<DT> ra_[DEPTH], rb_[DEPTH];
+    auto fetch = [&](int i, RawQuad<DT>& qa, RawQuad<DT>& qb) {
+        const WinGeom w = window_geom<false>(p, (task * ROUNDS + i / WPW) * WPW + i % WPW);
+        const size_t idx = (size_t)(w.y0 + srow) * w.pitch + (size_t)(w.x0 + scol);
+        qa.load(w.a, idx);
+        qb.load(w.b, idx);
+    };
+    sfor<DEPTH>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        fetch(i, ra_[i], rb_[i]);
+    });
+
+    sfor<PER>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+
+        // ---- stage: lane holds row srow, columns scol .. scol + 3 ----
+        {
+            float av[4], bv[5];
+            sfor<4>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                av[j] = ra_[i % DEPTH].get(j);
+                bv[j] = rb_[i % DEPTH].get(j);
+            });
+            if constexpr (i + DEPTH < PER) fetch(i + DEPTH, ra_[i % DEPTH], rb_[i % DEPTH]);
+            bv[4] = from_quad_next(bv[0]);      // b[row][(scol + 4) % 16]
+            uint32_t ta[4][NP], tb[5][NP];
+            sfor<4>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                split<NP>(av[j], ta[j]);
+            });
+            sfor<5>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                split<NP>(bv[j], tb[j]);
+            });
+            sfor<NP>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                const uint2 pa = make_uint2(ta[0][k] | (ta[1][k] << 16), ta[2][k] | (ta[3][k] << 16));
+                const uint2 pe = make_uint2(tb[0][k] | (tb[1][k] << 16), tb[2][k] | (tb[3][k] << 16));
+                const uint2 po = make_uint2(tb[1][k] | (tb[2][k] << 16), tb[3][k] | (tb[4][k] << 16));
+                *reinterpret_cast<uint2*>(&lds_a[k][srow][scol]) = pa;
+                // copy 0: E[m] = b[m % 16]; copy 1: O[m] = b[(m + 1) % 16]
+                *reinterpret_cast<uint2*>(&lds_b[k][0][srow][scol]) = pe;
+                *reinterpret_cast<uint2*>(&lds_b[k][0][srow][scol + W]) = pe;
+                *reinterpret_cast<uint2*>(&lds_b[k][1][srow][scol]) = po;
+                *reinterpret_cast<uint2*>(&lds_b[k][1][srow][scol + W]) = po;
+            });
+        }
+        wave_sync();
+
+        // ---- C = P Q on MFMA: 8 k-chunks of 32 ----
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        sfor<8>([&](auto Cc) {
+            constexpr int c = decltype(Cc)::value;
+            const int ra = (2 * c + (g >> 1) - i16) & (W - 1);
+            const int rb = 2 * c + (g >> 1);
+            const int o = 8 * (g & 1) + i16;            // first column of the B run
+            const int copy = o & 1, base = o & ~1;
+            bf16x8 A[NP], B[NP];
+            sfor<NP>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                A[k] = *reinterpret_cast<const bf16x8*>(&lds_a[k][ra][8 * (g & 1)]);
+                const uint32_t* q = reinterpret_cast<const uint32_t*>(&lds_b[k][copy][rb][base]);
+                uint32_t w4[4] = {q[0], q[1], q[2], q[3]};
+                B[k] = __builtin_bit_cast(bf16x8, w4);
+            });
+            // cross products x_i y_j (u16: all four; f32: i + j <= 2), smallest first
+            if constexpr (NP == 1) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[0], acc, 0, 0, 0);
+            } else if constexpr (NP == 2) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[0], acc, 0, 0, 0);
+            } else {
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[2], B[0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[2], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[1], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[0], acc, 0, 0, 0);
+            }
+        });
+
+        accs[i] = acc;
+        wave_sync();                            // staging buffers free for the next window
+    });
+
+    // D[sy][sx]: sx = i16, sy = 4 g + r.  The reference's plane is the
+    // un-normalised inverse FFT, W^2 times C (an exact power of two), and the
+    // quadrant swap puts C[sy][sx] at plane (sy + 8) % 16, (sx + 8) % 16.
+    const int wcol = (i16 + 8) & (W - 1);
+    sfor<PER>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        sfor<4>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            lds_p[i][(4 * g + r + 8) & (W - 1)][wcol] = accs[i][r] * (float)(W * W);
+        });
+    });
+    wave_sync();
+
+    // ---- column t of this lane group's two planes, then the common tail ----
+    float s[ROUNDS][W];
+    sfor<ROUNDS>([&](auto R) {
+        constexpr int r = decltype(R)::value;
+        sfor<W>([&](auto Hh) {
+            constexpr int h = decltype(Hh)::value;
+            s[r][h] = lds_p[r * WPW + slot][h][t];
+        });
+    });
+    finish_windows<W, ROUNDS, PLANES>(p, s, geo, active, lane, t);
+}
+
+template <int DT>
+static hipError_t launch_direct_dt(const CorrParams& p, bool planes, hipStream_t stream)
+{
+    const int blocks = (p.n_windows + 4 * 8 - 1) / (4 * 8);   // 4 waves x 8 windows
+    if (planes)
+        hipLaunchKernelGGL((corr_direct16_kernel<DT, true>), dim3(blocks), dim3(256), 0, stream, p);
+    else
+        hipLaunchKernelGGL((corr_direct16_kernel<DT, false>), dim3(blocks), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+
+bool direct_supported(int win) { return win == 16; }
+
+hipError_t launch_corr_direct(const CorrParams& p, int win, int dtype, bool planes, hipStream_t stream)
+{
+    if (p.n_windows <= 0) return hipSuccess;
+    if (win != 16 || p.real_compat) return hipErrorInvalidValue;
+    switch (dtype) {
+    case 0: return launch_direct_dt<0>(p, planes, stream);
+    case 1: return launch_direct_dt<1>(p, planes, stream);
+    case 2: return launch_direct_dt<2>(p, planes, stream);
+    case 3: return launch_direct_dt<3>(p, planes, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace opiv

@@ -31,6 +31,8 @@ DTYPE_CODES = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16,
 # opiv_variant
 CIRCULAR_FFT_FP32 = 0
 REAL_COMPAT = 1
+DIRECT_BF16_SPLIT = 2
+AUTO = 3
 # opiv_flag
 FLAG_FEWER_THAN_2_PEAKS = 1
 FLAG_NONFINITE_FIT = 2
@@ -156,6 +158,12 @@ def _check(rc: int) -> None:
     if rc != 0:
         msg = load().opiv_last_error().decode()
         raise OpivError(rc, msg)
+
+
+def direct_chosen(win: int) -> bool:
+    """Whether OPIV_AUTO picks the direct MFMA path for this window size
+    (it does where that path exists: 16 x 16)."""
+    return win == 16
 
 
 def version() -> str:
