@@ -151,19 +151,19 @@ def copy_calibration(mib: int, reps: int):
     print(json.dumps({"copy_calibration_mib": mib, "reps": reps}), flush=True)
 
 
-def bench_adjoint(steps: int, warmup: int):
-    """1024^2 f32 pair, uniform shift (1.6, -0.8), 32^2 windows at step 16:
+def bench_adjoint(steps: int, warmup: int, size: int = 1024):
+    """size^2 f32 pair, uniform shift (1.6, -0.8), 32^2 windows at step 16:
     device time of the forward pass, of the VJP (both frames' gradients of
     sum cot . (u, v)) and of the variance propagation (sigma = 2)."""
     import openpiv_amd as oa
-    a, b = oa.synth_pair(1024, 1024, seed=5, params=(1.6, -0.8), dtype=np.float32)
+    a, b = oa.synth_pair(size, size, seed=5, params=(1.6, -0.8), dtype=np.float32)
     ctx = oa.Context(0)
-    plan = oa.Plan(ctx, 1024, 1024, 32, dtype=np.float32)
+    plan = oa.Plan(ctx, size, size, 32, dtype=np.float32)
     n = plan.count
     da, db = oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)
     dfwd = oa.DeviceBuffer(n * RESULT_BYTES)
     dcot = oa.DeviceBuffer.from_array(np.random.default_rng(0).normal(size=(n, 2)))
-    dga, dgb = oa.DeviceBuffer(1024 * 1024 * 4), oa.DeviceBuffer(1024 * 1024 * 4)
+    dga, dgb = oa.DeviceBuffer(size * size * 4), oa.DeviceBuffer(size * size * 4)
     dvar = oa.DeviceBuffer(n * 16)
     lib = oa.load()
     ev = HipEvents()
@@ -185,10 +185,10 @@ def bench_adjoint(steps: int, warmup: int):
                                                         dga.ptr, dgb.ptr)))
     var_ms = timed(lambda: oa._check(lib.opiv_variance_pair(plan.handle, da.ptr, db.ptr, dfwd.ptr, 2.0,
                                                              dvar.ptr)))
-    line = {"metric": "adjoint cost relative to the forward pass (1024^2 f32, 32^2 windows, step 16)",
+    line = {"metric": f"adjoint cost relative to the forward pass ({size}^2 f32, 32^2 windows, step 16)",
             "value": round(vjp_ms / fwd_ms, 3), "unit": "x forward", "n_gpus": 1, "steps": steps,
             "higher_is_better": False, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "1024x1024 f32 pair, uniform shift (1.6, -0.8), 32x32 windows step 16",
+            "config": {"workload": f"{size}x{size} f32 pair, uniform shift (1.6, -0.8), 32x32 windows step 16",
                        "windows": n},
             "forward_ms": round(fwd_ms, 4), "vjp_ms": round(vjp_ms, 4), "variance_ms": round(var_ms, 4),
             "variance_over_forward": round(var_ms / fwd_ms, 3),
@@ -272,7 +272,8 @@ def main():
                     help="correlation path: fused FP32 FFT, BF16-split MFMA direct (16^2), or the faster")
     ap.add_argument("--dtype", choices=["f32", "u16", "u8"], default="f32")
     ap.add_argument("--adjoint", action="store_true",
-                    help="1024^2 f32 differentiation workload: forward vs VJP vs variance (prints its own line)")
+                    help="f32 differentiation workload: forward vs VJP vs variance (prints its own line)")
+    ap.add_argument("--adjoint-size", type=int, default=1024, help="frame side of the --adjoint workload")
     ap.add_argument("--batch", action="store_true",
                     help="24 uneven frame pairs x passes 64/32/16 in one graph launch (prints its own line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -292,7 +293,7 @@ def main():
         bench_batch(args.steps, args.warmup)
         return
     if args.adjoint:
-        bench_adjoint(args.steps, args.warmup)
+        bench_adjoint(args.steps, args.warmup, args.adjoint_size)
         return
 
     win = args.win

@@ -17,6 +17,12 @@
 // every tap is a register read.  The five C_k are recomputed as direct dot
 // products with FP64 accumulation, so the fit's derivatives use FP64 logs
 // of accurate values.
+//
+// The VJP runs in two launches: the window kernel stops at the five tap
+// weights (cotangent folded in), and a gather kernel with one thread per
+// frame pixel sums, over the windows that cover the pixel, the weighted taps
+// of the other frame (wrapped inside the window).  Each gradient element is
+// written once: no atomics and no clearing pass.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -175,6 +181,19 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjParams p)
             }
         });
     }
+    if constexpr (!VARIANCE) {
+        if (active && t == 0) {
+            AdjCoef c;
+            sfor<5>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                c.f[k] = valid ? fu[k] : 0.f;
+            });
+            c.shift = sx0 | (sy0 << 16);
+            c.pad_[0] = c.pad_[1] = 0;
+            p.coef[local] = c;
+        }
+        return;
+    }
 
     // ---- gradients: dC_k/da[n] = b[n + s_k], dC_k/db[m] = a[m - s_k] ----
     // g[0][y] (frame A) and g[1][y] (frame B) of column t, for u (or the
@@ -212,33 +231,67 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjParams p)
                 }
             });
         });
-        if constexpr (VARIANCE) {
-            sfor<W>([&](auto Y) {
-                constexpr int y = decltype(Y)::value;
-                su = fma((double)g_u[y], (double)g_u[y], su);
-                sv = fma((double)g_v[y], (double)g_v[y], sv);
-            });
-        } else if (valid) {
-            float* g = (fr == 0 ? p.grad_a : p.grad_b) + (size_t)y0 * p.gpitch + (size_t)(x0 + t);
-            sfor<W>([&](auto Y) {
-                constexpr int y = decltype(Y)::value;
-                atomicAdd(g + (size_t)y * p.gpitch, g_u[y]);
-            });
-        }
+        sfor<W>([&](auto Y) {
+            constexpr int y = decltype(Y)::value;
+            su = fma((double)g_u[y], (double)g_u[y], su);
+            sv = fma((double)g_v[y], (double)g_v[y], sv);
+        });
     });
 
-    if constexpr (VARIANCE) {
 #pragma unroll
-        for (int off = W / 2; off >= 1; off >>= 1) {
-            su += __shfl_xor(su, off, 64);
-            sv += __shfl_xor(sv, off, 64);
-        }
-        if (active && t == 0) {
-            const double s2 = p.sigma * p.sigma;
-            p.var[2 * (size_t)local] = valid ? s2 * su : 0.0;
-            p.var[2 * (size_t)local + 1] = valid ? s2 * sv : 0.0;
+    for (int off = W / 2; off >= 1; off >>= 1) {
+        su += __shfl_xor(su, off, 64);
+        sv += __shfl_xor(sv, off, 64);
+    }
+    if (active && t == 0) {
+        const double s2 = p.sigma * p.sigma;
+        p.var[2 * (size_t)local] = valid ? s2 * su : 0.0;
+        p.var[2 * (size_t)local + 1] = valid ? s2 * sv : 0.0;
+    }
+}
+
+// VJP gather: thread (x, y) of the gradient frames (x < gpitch, y < img_h)
+// sums, over the windows (ix, iy) whose square covers it, the five weighted
+// taps: grad_a[n] += f_k b[n + s_k], grad_b[n] += f_k a[n - s_k], positions
+// wrapped inside the window.  Pixels outside every window get zero.
+template <int W, int DT>
+__global__ __launch_bounds__(256) void adjoint_gather_kernel(AdjParams p)
+{
+    constexpr int M = W - 1;
+    constexpr int TDY[5] = {0, 0, 0, -1, 1};
+    constexpr int TDX[5] = {-1, 0, 1, 0, 0};
+    const int x = blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    if (x >= (int)p.gpitch || y >= p.img_h) return;
+
+    // windows ix with x0(ix) <= x < x0(ix) + W, likewise iy
+    const int dx = x - p.x_start, dy = y - p.y_start;
+    const int ix_lo = dx < W ? 0 : (dx - W) / p.step_x + 1;
+    const int ix_hi = dx < 0 ? -1 : min(dx / p.step_x, p.nx - 1);
+    const int iy_lo = dy < W ? 0 : (dy - W) / p.step_y + 1;
+    const int iy_hi = dy < 0 ? -1 : min(dy / p.step_y, p.ny - 1);
+
+    float ga = 0.f, gb = 0.f;
+    for (int iy = iy_lo; iy <= iy_hi; ++iy) {
+        const int y0 = p.y_start + iy * p.step_y;
+        const int ly = y - y0;
+        for (int ix = ix_lo; ix <= ix_hi; ++ix) {
+            const AdjCoef c = p.coef[(size_t)iy * p.nx + ix];
+            const int x0 = p.x_start + ix * p.step_x;
+            const int lx = x - x0;
+            const int sx0 = c.shift & 0xffff, sy0 = c.shift >> 16;
+            sfor<5>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                const int sy = sy0 + TDY[k], sx = sx0 + TDX[k];
+                const size_t ib = (size_t)(y0 + ((ly + sy) & M)) * p.pitch + (size_t)(x0 + ((lx + sx) & M));
+                const size_t ia = (size_t)(y0 + ((ly - sy) & M)) * p.pitch + (size_t)(x0 + ((lx - sx) & M));
+                ga = fmaf(c.f[k], load_px<DT>(p.b, ib), ga);
+                gb = fmaf(c.f[k], load_px<DT>(p.a, ia), gb);
+            });
         }
     }
+    p.grad_a[(size_t)y * p.gpitch + x] = ga;
+    p.grad_b[(size_t)y * p.gpitch + x] = gb;
 }
 
 template <int W, int DT>
@@ -246,10 +299,15 @@ static hipError_t launch_adj_w(const AdjParams& p, bool variance, hipStream_t st
 {
     constexpr int per_block = 4 * (64 / W);
     const int blocks = (p.n_windows + per_block - 1) / per_block;
-    if (variance)
+    if (variance) {
         hipLaunchKernelGGL((adjoint_kernel<W, DT, true>), dim3(blocks), dim3(256), 0, stream, p);
-    else
-        hipLaunchKernelGGL((adjoint_kernel<W, DT, false>), dim3(blocks), dim3(256), 0, stream, p);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((adjoint_kernel<W, DT, false>), dim3(blocks), dim3(256), 0, stream, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const dim3 grid((unsigned)((p.gpitch + 63) / 64), (unsigned)((p.img_h + 3) / 4));
+    hipLaunchKernelGGL((adjoint_gather_kernel<W, DT>), grid, dim3(256), 0, stream, p);
     return hipGetLastError();
 }
 
@@ -268,6 +326,9 @@ static hipError_t launch_adj_dt(const AdjParams& p, int win, bool variance, hipS
 hipError_t launch_adjoint(const AdjParams& p, int win, int dtype, bool variance, hipStream_t stream)
 {
     if (p.n_windows <= 0) return hipSuccess;
+    if (!variance && (p.coef == nullptr || p.nx <= 0 || p.ny <= 0 || p.step_x <= 0 || p.step_y <= 0 ||
+                      p.img_h <= 0 || (long long)p.nx * p.ny != p.n_windows))
+        return hipErrorInvalidValue;
     switch (dtype) {
     case 0: return launch_adj_dt<0>(p, win, variance, stream);
     case 1: return launch_adj_dt<1>(p, win, variance, stream);

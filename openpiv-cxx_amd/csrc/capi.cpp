@@ -60,7 +60,7 @@ int opiv_ctx_destroy(opiv_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto* b : { &ctx->stage_a, &ctx->stage_b, &ctx->stage_out, &ctx->stage_fwd, &ctx->stage_cot,
-                     &ctx->stage_ga, &ctx->stage_gb, &ctx->stage_var })
+                     &ctx->stage_ga, &ctx->stage_gb, &ctx->stage_var, &ctx->adj_coef })
         b->release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -369,7 +369,9 @@ static int adjoint_setup(opiv_plan* plan, const void* a, const void* b, const op
     p.step_x = (int32_t)plan->off_x;
     p.step_y = (int32_t)plan->off_y;
     p.nx = (int32_t)plan->nx;
+    p.ny = (int32_t)plan->ny;
     p.n_windows = (int32_t)count;
+    p.img_h = (int32_t)plan->desc.img_h;
     p.fwd = (const opiv_window_result*)dfwd;
     return OPIV_OK;
 }
@@ -398,8 +400,8 @@ int opiv_vjp_pair(opiv_plan* plan, const void* a, const void* b, const opiv_wind
         HIP_TRY(ctx->stage_gb.ensure(gbytes));
         dgb = (float*)ctx->stage_gb.p;
     }
-    HIP_TRY(hipMemsetAsync(dga, 0, gbytes, ctx->stream));
-    HIP_TRY(hipMemsetAsync(dgb, 0, gbytes, ctx->stream));
+    HIP_TRY(ctx->adj_coef.ensure(count * sizeof(opiv::AdjCoef)));
+    p.coef = (opiv::AdjCoef*)ctx->adj_coef.p;
     p.cot = (const double*)dcot;
     p.grad_a = dga;
     p.grad_b = dgb;

@@ -30,14 +30,49 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // value of lane+1 / lane-1 (whole-wave DPP shifts; the group-edge lanes
-// receive a neighbour group's value, which the scan never uses)
+// receive a neighbour group's value or zero, which the scan never uses)
 __device__ __forceinline__ float from_right(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_left(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+// Partner exchange for reductions over lane groups of W lanes, applied with
+// OFF = 1, 2, 4, ... W/2 in that order: lane ^ 1 and lane ^ 2 (quad_perm),
+// the mirror lane of the 8-lane half (row_half_mirror), the mirror lane of the
+// 16-lane row (row_mirror), then lane ^ OFF.  Each step pairs lanes whose
+// accumulated sets are disjoint, and after the last one every lane holds its
+// whole group.
+template <int OFF>
+__device__ __forceinline__ int group_partner(int v)
+{
+    if constexpr (OFF == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xf, 0xf, false);
+    else if constexpr (OFF == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xf, 0xf, false);
+    else if constexpr (OFF == 4) return __builtin_amdgcn_mov_dpp(v, 0x141, 0xf, 0xf, false);
+    else if constexpr (OFF == 8) return __builtin_amdgcn_mov_dpp(v, 0x140, 0xf, 0xf, false);
+    else return __shfl_xor(v, OFF, 64);
+}
+template <int OFF>
+__device__ __forceinline__ float group_partner(float v)
+{
+    return __int_as_float(group_partner<OFF>(__float_as_int(v)));
+}
+
+// value of lane + D of the same 16-lane row (row_ror by 16 - D); the caller
+// keeps lane % 16 + D below 16
+template <int D>
+__device__ __forceinline__ double from_row_ahead(double v)
+{
+    if constexpr (D == 0) return v;
+    else {
+        const int2 w = __builtin_bit_cast(int2, v);
+        const int lo = __builtin_amdgcn_mov_dpp(w.x, 0x120 + (16 - D), 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_mov_dpp(w.y, 0x120 + (16 - D), 0xf, 0xf, false);
+        return __builtin_bit_cast(double, make_int2(lo, hi));
+    }
 }
 
 // frames are global memory: address-space-1 pointers give global (not flat)
@@ -53,6 +88,8 @@ __device__ __forceinline__ float load_px(const void* base, size_t idx)
     else if constexpr (DT == 2) return ((gptr<float>)base)[idx];
     else return (float)((gptr<double>)base)[idx];
 }
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
 // ordering of find_peaks' sort (descending value); equal values keep scan
 // order, i.e. the smaller row-major index comes first
@@ -150,16 +187,28 @@ __device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, i
         v1 = gt ? mc : v1;
     });
     i1 = h1 * W + t;
-#pragma unroll
-    for (int off = W / 2; off >= 1; off >>= 1) {
-        const float ov1 = __shfl_xor(v1, off, 64);
-        const float ov2 = __shfl_xor(v2, off, 64);
-        const int oi1 = __shfl_xor(i1, off, 64);
+    // top-2 merge of disjoint lane sets, ascending partner distance
+    sfor<ilog2(W)>([&](auto K) {
+        constexpr int off = 1 << decltype(K)::value;
+        const float ov1 = group_partner<off>(v1);
+        const float ov2 = group_partner<off>(v2);
+        const int oi1 = group_partner<off>(i1);
         v2 = fmaxf(fmaxf(v2, ov2), fminf(v1, ov1));
         const bool mine = better(v1, i1, ov1, oi1);
         v1 = mine ? v1 : ov1;
         i1 = mine ? i1 : oi1;
-    }
+    });
+}
+
+// max over the lanes of each W-lane group
+template <int W>
+__device__ __forceinline__ float group_max(float v)
+{
+    sfor<ilog2(W)>([&](auto K) {
+        constexpr int off = 1 << decltype(K)::value;
+        v = fmaxf(v, group_partner<off>(v));
+    });
+    return v;
 }
 
 // Results of the ROUNDS windows of this lane group from their swapped planes
@@ -218,7 +267,8 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
             st[r][4] = __shfl(dn, gbase + px[r], 64);
         });
 
-        // ---- FP64 logs of the 5 * ROUNDS stencil values, one per lane ----
+        // ---- FP64 logs of the 5 * ROUNDS stencil values, one per lane:
+        // value j of round r sits on lane j * ROUNDS + r of the group ----
         constexpr int NV = 5 * ROUNDS;
         constexpr int NP = (NV + W - 1) / W;
         double lg[NP];
@@ -226,57 +276,94 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
             constexpr int k = decltype(K)::value;
             float v = 1.f;
             sfor<NV>([&](auto J) {
-                constexpr int j = decltype(J)::value;
-                v = (t + k * W == j) ? st[j / 5][j % 5] : v;
+                constexpr int L = decltype(J)::value;
+                v = (t + k * W == L) ? st[L % ROUNDS][L / ROUNDS] : v;
             });
             lg[k] = log((double)v);
         });
-        double ln[NV];
-        sfor<NV>([&](auto J) {
-            constexpr int j = decltype(J)::value;
-            ln[j] = __shfl(lg[j / W], gbase + (j % W), 64);
-        });
 
+        // ---- lane t < ROUNDS of the group fits window t ----
+        const int rr = t < ROUNDS ? t : 0;
+        auto pick = [&](const auto (&arr)[ROUNDS]) {
+            auto v = arr[0];
+            sfor<ROUNDS - 1>([&](auto R) {
+                constexpr int r = decltype(R)::value + 1;
+                v = rr == r ? arr[r] : v;
+            });
+            return v;
+        };
+        // logs of round rr: value j from lane rr + j * ROUNDS
+        double ln[5];
+        if constexpr (W >= 16 && NV <= W) {
+            sfor<5>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                ln[j] = from_row_ahead<j * ROUNDS>(lg[0]);
+            });
+        } else {
+            sfor<5>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                const int L = j * ROUNDS + rr;
+                double v = 0.0;
+                sfor<NP>([&](auto K) {
+                    constexpr int k = decltype(K)::value;
+                    const double x = __shfl(lg[k], gbase + L % W, 64);
+                    v = L / W == k ? x : v;
+                });
+                ln[j] = v;
+            });
+        }
+        bool act[ROUNDS], tw[ROUNDS];
+        float nmx[ROUNDS];
+        int gx0[ROUNDS], gy0[ROUNDS], gout[ROUNDS];
+        uint32_t gh[ROUNDS];
         sfor<ROUNDS>([&](auto R) {
             constexpr int r = decltype(R)::value;
-            if (!(active[r] && t == r)) return;
-            opiv_window_result res;
-            res.reserved = 0;
-            if (!two[r]) {
-                // main.cpp:214-218: the default (all-zero) record stays
-                res.x = res.y = res.u = res.v = res.sn = 0.0;
-                res.peak0 = res.peak1 = 0.0;
-                res.px = res.py = 0;
-                res.flags = OPIV_FLAG_FEWER_THAN_2_PEAKS;
-            } else {
-                // image_utils.impl.h:86-98 on the logs of (l, c, r) and (up, c, down)
-                const double lnl = ln[5 * r + 0], lnc = ln[5 * r + 1], lnr = ln[5 * r + 2];
-                const double lnu = ln[5 * r + 3], lnd = ln[5 * r + 4];
-                const double numx = lnl - lnr, denx = 2.0 * (lnl + lnr - 2.0 * lnc);
-                const double numy = lnu - lnd, deny = 2.0 * (lnu + lnd - 2.0 * lnc);
-                const double plx = (double)px[r] + (denx == 0.0 ? 0.0 : numx / denx);
-                const double ply = (double)py[r] + (deny == 0.0 ? 0.0 : numy / deny);
-                uint32_t flags = 0;
-                if (!isfinite(plx) || !isfinite(ply)) flags |= OPIV_FLAG_NONFINITE_FIT;
-                const float nmax = fmaxf(fmaxf(st[r][0], st[r][2]), fmaxf(st[r][3], st[r][4]));
-                const float tol = p.tie_tol * fabsf(v1[r]);
-                if (v1[r] - v2[r] <= tol || v1[r] - nmax <= tol) flags |= OPIV_FLAG_NEAR_TIE;
-                if (v1[r] == v2[r]) flags |= OPIV_FLAG_EXACT_TIE;
-                // main.cpp:220-234
-                const int midx = geo[r].x0 + W / 2, midy = geo[r].y0 + W / 2;
-                res.x = (double)midx;
-                res.y = (double)geo[r].img_h - (double)midy;
-                res.u = (double)midx - ((double)geo[r].x0 + plx);
-                res.v = (double)midy - ((double)geo[r].y0 + ply);
-                res.sn = (v2[r] > 0.f) ? (double)v1[r] / (double)v2[r] : 0.0;
-                res.peak0 = (double)v1[r];
-                res.peak1 = (double)v2[r];
-                res.px = px[r];
-                res.py = py[r];
-                res.flags = flags;
-            }
-            p.out[geo[r].out] = res;
+            act[r] = active[r];
+            tw[r] = two[r];
+            nmx[r] = fmaxf(fmaxf(st[r][0], st[r][2]), fmaxf(st[r][3], st[r][4]));
+            gx0[r] = geo[r].x0;
+            gy0[r] = geo[r].y0;
+            gout[r] = geo[r].out;
+            gh[r] = geo[r].img_h;
         });
+        if (t >= ROUNDS || !pick(act)) return;
+        const float fv1 = pick(v1), fv2 = pick(v2), nmax = pick(nmx);
+        const int fpx = pick(px), fpy = pick(py);
+        const int x0 = pick(gx0), y0 = pick(gy0);
+        opiv_window_result res;
+        res.reserved = 0;
+        if (!pick(tw)) {
+            // main.cpp:214-218: the default (all-zero) record stays
+            res.x = res.y = res.u = res.v = res.sn = 0.0;
+            res.peak0 = res.peak1 = 0.0;
+            res.px = res.py = 0;
+            res.flags = OPIV_FLAG_FEWER_THAN_2_PEAKS;
+        } else {
+            // image_utils.impl.h:86-98 on the logs of (l, c, r) and (up, c, down)
+            const double lnl = ln[0], lnc = ln[1], lnr = ln[2], lnu = ln[3], lnd = ln[4];
+            const double numx = lnl - lnr, denx = 2.0 * (lnl + lnr - 2.0 * lnc);
+            const double numy = lnu - lnd, deny = 2.0 * (lnu + lnd - 2.0 * lnc);
+            const double plx = (double)fpx + (denx == 0.0 ? 0.0 : numx / denx);
+            const double ply = (double)fpy + (deny == 0.0 ? 0.0 : numy / deny);
+            uint32_t flags = 0;
+            if (!isfinite(plx) || !isfinite(ply)) flags |= OPIV_FLAG_NONFINITE_FIT;
+            const float tol = p.tie_tol * fabsf(fv1);
+            if (fv1 - fv2 <= tol || fv1 - nmax <= tol) flags |= OPIV_FLAG_NEAR_TIE;
+            if (fv1 == fv2) flags |= OPIV_FLAG_EXACT_TIE;
+            // main.cpp:220-234
+            const int midx = x0 + W / 2, midy = y0 + W / 2;
+            res.x = (double)midx;
+            res.y = (double)pick(gh) - (double)midy;
+            res.u = (double)midx - ((double)x0 + plx);
+            res.v = (double)midy - ((double)y0 + ply);
+            res.sn = (fv2 > 0.f) ? (double)fv1 / (double)fv2 : 0.0;
+            res.peak0 = (double)fv1;
+            res.peak1 = (double)fv2;
+            res.px = fpx;
+            res.py = fpy;
+            res.flags = flags;
+        }
+        p.out[pick(gout)] = res;
     }
 }
 

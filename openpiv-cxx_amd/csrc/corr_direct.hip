@@ -115,8 +115,12 @@ __global__ __launch_bounds__(256) void corr_direct16_kernel(CorrParams p)
     constexpr int ROUNDS = 2, WPW = 4, PER = ROUNDS * WPW;
     // per wave: the operand staging of one window (rows of a; doubled rows of
     // b, two copies), later overlaid by the 8 swapped planes
-    typedef uint16_t StageA[NP][W][W];
-    typedef uint16_t StageB[NP][2][W][2 * W];
+    // A rows padded to 12 dwords: the 8 rows of a b128 read group start in
+    // distinct bank quads.  The odd-shift copy of B sits 16 dwords off the
+    // even one, so that even and odd lanes of a read hit different banks.
+    constexpr int APITCH = 24, BCOPY = 2 * W * W + 32;
+    typedef uint16_t StageA[NP][W][APITCH];
+    typedef uint16_t StageB[NP][2][BCOPY];
     typedef float Planes[PER][W][W + 1];
     constexpr int STAGE = (int)(sizeof(StageA) + sizeof(StageB));
     constexpr int BYTES = STAGE > (int)sizeof(Planes) ? STAGE : (int)sizeof(Planes);
@@ -149,15 +153,20 @@ __global__ __launch_bounds__(256) void corr_direct16_kernel(CorrParams p)
     // raw pixels of the next windows, loaded DEPTH windows ahead of their staging
     constexpr int DEPTH = DT <= 1 ? PER : (DT == 2 ? 4 : 2);
     RawQuad<DT> ra_[DEPTH], rb_[DEPTH];
-    auto fetch = [&](int i, RawQuad<DT>& qa, RawQuad<DT>& qb) {
-        const WinGeom w = window_geom<false>(p, (task * ROUNDS + i / WPW) * WPW + i % WPW);
-        const size_t idx = (size_t)(w.y0 + srow) * w.pitch + (size_t)(w.x0 + scol);
-        qa.load(w.a, idx);
-        qb.load(w.b, idx);
+    // window i = (round i / 4, slot i % 4): its corner is that lane group's
+    // geometry, read into scalars
+    const size_t lane_off = (size_t)srow * p.pitch + (size_t)scol;
+    auto fetch = [&](auto I, RawQuad<DT>& qa, RawQuad<DT>& qb) {
+        constexpr int i = decltype(I)::value;
+        const int x0 = __builtin_amdgcn_readlane(geo[i / WPW].x0, (i % WPW) * W);
+        const int y0 = __builtin_amdgcn_readlane(geo[i / WPW].y0, (i % WPW) * W);
+        const size_t idx = (size_t)y0 * p.pitch + (size_t)x0 + lane_off;
+        qa.load(p.a, idx);
+        qb.load(p.b, idx);
     };
     sfor<DEPTH>([&](auto I) {
         constexpr int i = decltype(I)::value;
-        fetch(i, ra_[i], rb_[i]);
+        fetch(I, ra_[i], rb_[i]);
     });
 
     sfor<PER>([&](auto I) {
@@ -171,7 +180,8 @@ __global__ __launch_bounds__(256) void corr_direct16_kernel(CorrParams p)
                 av[j] = ra_[i % DEPTH].get(j);
                 bv[j] = rb_[i % DEPTH].get(j);
             });
-            if constexpr (i + DEPTH < PER) fetch(i + DEPTH, ra_[i % DEPTH], rb_[i % DEPTH]);
+            if constexpr (i + DEPTH < PER)
+                fetch(std::integral_constant<int, i + DEPTH>{}, ra_[i % DEPTH], rb_[i % DEPTH]);
             bv[4] = from_quad_next(bv[0]);      // b[row][(scol + 4) % 16]
             uint32_t ta[4][NP], tb[5][NP];
             sfor<4>([&](auto J) {
@@ -189,10 +199,10 @@ __global__ __launch_bounds__(256) void corr_direct16_kernel(CorrParams p)
                 const uint2 po = make_uint2(tb[1][k] | (tb[2][k] << 16), tb[3][k] | (tb[4][k] << 16));
                 *reinterpret_cast<uint2*>(&lds_a[k][srow][scol]) = pa;
                 // copy 0: E[m] = b[m % 16]; copy 1: O[m] = b[(m + 1) % 16]
-                *reinterpret_cast<uint2*>(&lds_b[k][0][srow][scol]) = pe;
-                *reinterpret_cast<uint2*>(&lds_b[k][0][srow][scol + W]) = pe;
-                *reinterpret_cast<uint2*>(&lds_b[k][1][srow][scol]) = po;
-                *reinterpret_cast<uint2*>(&lds_b[k][1][srow][scol + W]) = po;
+                *reinterpret_cast<uint2*>(&lds_b[k][0][srow * 2 * W + scol]) = pe;
+                *reinterpret_cast<uint2*>(&lds_b[k][0][srow * 2 * W + scol + W]) = pe;
+                *reinterpret_cast<uint2*>(&lds_b[k][1][srow * 2 * W + scol]) = po;
+                *reinterpret_cast<uint2*>(&lds_b[k][1][srow * 2 * W + scol + W]) = po;
             });
         }
         wave_sync();
@@ -209,7 +219,7 @@ __global__ __launch_bounds__(256) void corr_direct16_kernel(CorrParams p)
             sfor<NP>([&](auto K) {
                 constexpr int k = decltype(K)::value;
                 A[k] = *reinterpret_cast<const bf16x8*>(&lds_a[k][ra][8 * (g & 1)]);
-                const uint32_t* q = reinterpret_cast<const uint32_t*>(&lds_b[k][copy][rb][base]);
+                const uint32_t* q = reinterpret_cast<const uint32_t*>(&lds_b[k][copy][rb * 2 * W + base]);
                 uint32_t w4[4] = {q[0], q[1], q[2], q[3]};
                 B[k] = __builtin_bit_cast(bf16x8, w4);
             });

@@ -92,8 +92,7 @@ __device__ __forceinline__ float forward_product(const WinGeom& g, bool real_com
         constexpr int k = decltype(K)::value;
         zmax = fmaxf(zmax, fmaxf(fabsf(z[k].x), fabsf(z[k].y)));
     });
-#pragma unroll
-    for (int off = W / 2; off >= 1; off >>= 1) zmax = fmaxf(zmax, __shfl_xor(zmax, off, 64));
+    zmax = group_max<W>(zmax);
     scale = 1.f;
     if constexpr (ACC) {
         int e0, e1;

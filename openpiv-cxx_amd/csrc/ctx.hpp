@@ -99,6 +99,7 @@ struct opiv_ctx {
     hipStream_t stream = nullptr;
     opiv::detail::DevBuf stage_a, stage_b, stage_out;
     opiv::detail::DevBuf stage_fwd, stage_cot, stage_ga, stage_gb, stage_var;
+    opiv::detail::DevBuf adj_coef;   // VJP tap weights, 24 B per window
 };
 
 struct opiv_plan {
