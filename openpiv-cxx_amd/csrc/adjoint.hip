@@ -80,14 +80,17 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjParams p)
     constexpr int WAVES = 256 / 64;
     constexpr int H2 = W / 2;
     constexpr int M = W - 1;
-    __shared__ float lds_all[WAVES][WPW][2][W][W + 1];
+    // frame B's window (both frames for the variance, whose gradient pass
+    // also reads frame A's columns)
+    constexpr int NF = VARIANCE ? 2 : 1;
+    __shared__ float lds_all[WAVES][WPW][NF][W][W + 1];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int slot = lane / W;
     const int t = lane % W;
-    float (*la)[W + 1] = lds_all[wave][slot][0];
-    float (*lb)[W + 1] = lds_all[wave][slot][1];
+    float (*lb)[W + 1] = lds_all[wave][slot][0];
+    float (*la)[W + 1] = lds_all[wave][slot][NF - 1];
 
     const int local = (blockIdx.x * WAVES + wave) * WPW + slot;
     const bool active = local < p.n_windows;
@@ -105,7 +108,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjParams p)
         sfor<W>([&](auto Y) {
             constexpr int y = decltype(Y)::value;
             acol[y] = load_px<DT>(p.a, idx + (size_t)y * p.pitch);
-            la[y][t] = acol[y];
+            if constexpr (VARIANCE) la[y][t] = acol[y];
             lb[y][t] = load_px<DT>(p.b, idx + (size_t)y * p.pitch);
         });
     }
@@ -294,6 +297,127 @@ __global__ __launch_bounds__(256) void adjoint_gather_kernel(AdjParams p)
     p.grad_b[(size_t)y * p.gpitch + x] = gb;
 }
 
+// VJP gather through LDS (W <= 32): block = a 64 x 64 tile of gradient
+// pixels.  The windows overlapping the tile span a region of at most
+// (62 + 2W)^2 pixels; it is staged in LDS one frame at a time (frame B for
+// grad_a, then frame A for grad_b), so every tap read is an LDS read, and
+// the tap weights of those windows (at most kTileWin) are staged once.  Same
+// sums, in the same order, as adjoint_gather_kernel.
+constexpr int kTile = 64;
+constexpr int kTileWin = 256;
+template <int W>
+constexpr int region_side() { return kTile - 2 + 2 * W; }
+// windows of one axis that can overlap a tile
+inline int tile_windows(int win, int step) { return (kTile + win - 2) / step + 1; }
+
+template <int W, int DT>
+__global__ __launch_bounds__(1024) void adjoint_gather_tiled_kernel(AdjParams p)
+{
+    constexpr int M = W - 1;
+    constexpr int RS = region_side<W>();
+    constexpr int R = kTile / 16;               // rows per thread: column c, rows rg * R .. + R - 1
+    __shared__ float reg[RS * RS];
+    __shared__ AdjCoef wc[kTileWin];
+
+    const int tx0 = blockIdx.x * kTile, ty0 = blockIdx.y * kTile;
+    // windows overlapping the tile: x0 in (tx0 - W, tx0 + kTile), likewise y0
+    const int ex = tx0 - p.x_start - W + 1, ey = ty0 - p.y_start - W + 1;
+    const int ixa = ex <= 0 ? 0 : (ex + p.step_x - 1) / p.step_x;
+    const int iya = ey <= 0 ? 0 : (ey + p.step_y - 1) / p.step_y;
+    const int fx = tx0 + kTile - 1 - p.x_start, fy = ty0 + kTile - 1 - p.y_start;
+    const int ixb = fx < 0 ? -1 : min(fx / p.step_x, p.nx - 1);
+    const int iyb = fy < 0 ? -1 : min(fy / p.step_y, p.ny - 1);
+    const bool any = ixa <= ixb && iya <= iyb;
+    const int rx0 = p.x_start + ixa * p.step_x, ry0 = p.y_start + iya * p.step_y;
+    const int rw = any ? (ixb - ixa) * p.step_x + W : 0;
+    const int rh = any ? (iyb - iya) * p.step_y + W : 0;
+
+    // tap weights of the tile's windows, wc[(iy - iya) * nwx + ix - ixa]
+    const int nwx = any ? ixb - ixa + 1 : 0, nwy = any ? iyb - iya + 1 : 0;
+    for (int i = (int)threadIdx.x; i < nwx * nwy; i += 1024) {
+        const int wy = i / nwx, wx = i - wy * nwx;
+        wc[i] = p.coef[(size_t)(iya + wy) * p.nx + ixa + wx];
+    }
+
+    const int c = (int)(threadIdx.x & 63), rg = (int)(threadIdx.x >> 6);
+    const int x = tx0 + c, ya = ty0 + rg * R;
+    const int dx = x - p.x_start;
+    const int ix_lo = dx < W ? 0 : (dx - W) / p.step_x + 1;
+    const int ix_hi = dx < 0 ? -1 : min(dx / p.step_x, p.nx - 1);
+    // windows iy covering any of rows ya .. ya + R - 1
+    const int dya = ya - p.y_start, dyb = ya + R - 1 - p.y_start;
+    const int iy_lo = dya < W ? 0 : (dya - W) / p.step_y + 1;
+    const int iy_hi = dyb < 0 ? -1 : min(dyb / p.step_y, p.ny - 1);
+    // staging: column sc of the region, rows sr, sr + 8, ...
+    const int sc = (int)(threadIdx.x & 127), sr = (int)(threadIdx.x >> 7);
+
+    sfor<2>([&](auto F) {
+        constexpr int fr = decltype(F)::value;       // 0: grad_a from frame B, 1: grad_b from frame A
+        const void* src = fr == 0 ? p.b : p.a;
+        float* dst = fr == 0 ? p.grad_a : p.grad_b;
+        __syncthreads();                             // previous frame's readers are done
+        if (sc < rw) {
+            const size_t base = (size_t)ry0 * p.pitch + (size_t)(rx0 + sc);
+            for (int r = sr; r < rh; r += 32) {
+                float v[4];
+                sfor<4>([&](auto U) {
+                    constexpr int u = decltype(U)::value;
+                    const int rr = r + 8 * u;
+                    v[u] = rr < rh ? load_px<DT>(src, base + (size_t)rr * p.pitch) : 0.f;
+                });
+                sfor<4>([&](auto U) {
+                    constexpr int u = decltype(U)::value;
+                    const int rr = r + 8 * u;
+                    if (rr < rh) reg[rr * RS + sc] = v[u];
+                });
+            }
+        }
+        __syncthreads();
+
+        float g[R];
+        sfor<R>([&](auto I) { g[decltype(I)::value] = 0.f; });
+        for (int iy = iy_lo; iy <= iy_hi; ++iy) {
+            const int y0 = p.y_start + iy * p.step_y;
+            const int ly0 = ya - y0;                 // local row of the thread's first row
+            for (int ix = ix_lo; ix <= ix_hi; ++ix) {
+                const AdjCoef cf = wc[(iy - iya) * nwx + ix - ixa];
+                const int x0 = p.x_start + ix * p.step_x;
+                const int lx = x - x0;
+                const int sx0 = cf.shift & 0xffff, sy0 = cf.shift >> 16;
+                // columns of the taps l, c, r (up and down use the centre one)
+                const int cx = fr == 0 ? ((lx + sx0) & M) : ((lx - sx0) & M);
+                const int cl = (cx + (fr == 0 ? -1 : 1)) & M, cr = (cx + (fr == 0 ? 1 : -1)) & M;
+                const int ob = (y0 - ry0) * RS + (x0 - rx0);
+                sfor<R>([&](auto I) {
+                    constexpr int i = decltype(I)::value;
+                    const int ly = ly0 + i;
+                    if (ly >= 0 && ly < W) {
+                        const int ry = fr == 0 ? ((ly + sy0) & M) : ((ly - sy0) & M);
+                        const int rm = (ry - 1) & M, rp = (ry + 1) & M;
+                        const int o = ob + ry * RS;
+                        // taps l, c, r, up, down; for frame A (grad_b) the shift is
+                        // negated, which swaps l <-> r and up <-> down
+                        float t = g[i];
+                        t = fmaf(cf.f[0], reg[o + cl], t);
+                        t = fmaf(cf.f[1], reg[o + cx], t);
+                        t = fmaf(cf.f[2], reg[o + cr], t);
+                        t = fmaf(cf.f[3], reg[ob + (fr == 0 ? rm : rp) * RS + cx], t);
+                        t = fmaf(cf.f[4], reg[ob + (fr == 0 ? rp : rm) * RS + cx], t);
+                        g[i] = t;
+                    }
+                });
+            }
+        }
+        if (x < (int)p.gpitch) {
+            sfor<R>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                const int y = ya + i;
+                if (y < p.img_h) dst[(size_t)y * p.gpitch + x] = g[i];
+            });
+        }
+    });
+}
+
 template <int W, int DT>
 static hipError_t launch_adj_w(const AdjParams& p, bool variance, hipStream_t stream)
 {
@@ -306,8 +430,14 @@ static hipError_t launch_adj_w(const AdjParams& p, bool variance, hipStream_t st
     hipLaunchKernelGGL((adjoint_kernel<W, DT, false>), dim3(blocks), dim3(256), 0, stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const dim3 grid((unsigned)((p.gpitch + 63) / 64), (unsigned)((p.img_h + 3) / 4));
-    hipLaunchKernelGGL((adjoint_gather_kernel<W, DT>), grid, dim3(256), 0, stream, p);
+    const int twx = tile_windows(W, p.step_x), twy = tile_windows(W, p.step_y);
+    if (region_side<W>() * region_side<W>() * 4 <= 65536 && twx * twy <= kTileWin) {
+        const dim3 grid((unsigned)((p.gpitch + kTile - 1) / kTile), (unsigned)((p.img_h + kTile - 1) / kTile));
+        hipLaunchKernelGGL((adjoint_gather_tiled_kernel<W, DT>), grid, dim3(1024), 0, stream, p);
+    } else {
+        const dim3 grid((unsigned)((p.gpitch + 63) / 64), (unsigned)((p.img_h + 3) / 4));
+        hipLaunchKernelGGL((adjoint_gather_kernel<W, DT>), grid, dim3(256), 0, stream, p);
+    }
     return hipGetLastError();
 }
 
