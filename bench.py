@@ -333,7 +333,7 @@ def main():
     n, plain_ms, wall, res = measure(oa.CIRCULAR_FFT_FP32)
     chosen = args.path
     if args.path == "auto":
-        chosen = "direct" if oa.direct_chosen(win) else "fft"
+        chosen = "direct" if oa.direct_chosen(win, np_dtype) else "fft"
     ms_per_step = plain_ms
     if chosen == "direct":
         log(f"[bench] timing {args.steps} steps (direct MFMA path)")
@@ -363,10 +363,11 @@ def main():
         "metric": METRIC, "value": round(value, 1), "unit": "vectors/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": args.dtype, "data": "synthetic",
+        # arithmetic of the correlation: FP32 FFT, or bf16 products accumulated in FP32
+        "dtype": "bf16" if chosen == "direct" else "f32", "data": "synthetic",
         "config": {"workload": f"4096x4096 {args.dtype} frame pair, {win}x{win} windows step {step} ({n} windows), "
                                f"correlation + find_peaks + Gaussian fit fused, chosen path {path_name}",
-                   "windows": n, "window": win, "step": step, "path": path_name,
+                   "windows": n, "window": win, "step": step, "path": path_name, "frame_dtype": args.dtype,
                    "frames_resident_in_hbm": True},
         "plain_gpu": {"path": "fft_fp32_fused", "ms_per_step": round(plain_ms, 5),
                       "value": round(n / (plain_ms * 1e-3), 1)},

@@ -56,7 +56,8 @@ typedef enum opiv_variant {
      * u8 exact, u16 exact products, f32/f64 three-term split; 16x16 windows */
     OPIV_DIRECT_BF16_SPLIT = 2,
     /* the faster of OPIV_CIRCULAR_FFT_FP32 and OPIV_DIRECT_BF16_SPLIT for
-     * the plan's window size and pixel type (same semantics) */
+     * the plan's window size and pixel type as measured on MI355X (same
+     * semantics); at present OPIV_CIRCULAR_FFT_FP32 for every size and type */
     OPIV_AUTO = 3
 } opiv_variant;
 

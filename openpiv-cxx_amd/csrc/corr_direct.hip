@@ -22,7 +22,8 @@
 //         products x_i y_j with i + j <= 2 (relative error ~2^-24, Henry,
 //         Tang & Heinecke, "Leveraging the bfloat16 AI datatype for
 //         higher-precision computations", ARITH 2019)
-// Products are accumulated smallest first.
+// Each cross product (and, for u8, each K-chunk parity) has its own FP32
+// accumulator; the partial sums are added smallest product first.
 //
 // Mapping: a wave runs 8 windows through the MFMA one after another, then
 // finishes all 8 with the tail shared with the FFT kernel (one FP64 log per
@@ -107,7 +108,7 @@ __device__ __forceinline__ float from_quad_next(float v)
 // quadrant-swapped plane in LDS; then lane (slot, t) reads column t of its
 // two windows' planes.
 template <int DT, bool PLANES>
-__global__ __launch_bounds__(256) void corr_direct16_kernel(CorrParams p)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4 : (DT == 2 ? 3 : 2)))) void corr_direct16_kernel(CorrParams p)
 {
     constexpr int W = 16;
     constexpr int NP = nparts<DT>();
@@ -169,46 +170,48 @@ __global__ __launch_bounds__(256) void corr_direct16_kernel(CorrParams p)
         fetch(I, ra_[i], rb_[i]);
     });
 
-    sfor<PER>([&](auto I) {
+    // ---- stage window i: lane holds row srow, columns scol .. scol + 3 ----
+    auto stage = [&](auto I) {
         constexpr int i = decltype(I)::value;
+        float av[4], bv[5];
+        sfor<4>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            av[j] = ra_[i % DEPTH].get(j);
+            bv[j] = rb_[i % DEPTH].get(j);
+        });
+        if constexpr (i + DEPTH < PER)
+            fetch(std::integral_constant<int, i + DEPTH>{}, ra_[i % DEPTH], rb_[i % DEPTH]);
+        bv[4] = from_quad_next(bv[0]);          // b[row][(scol + 4) % 16]
+        uint32_t ta[4][NP], tb[5][NP];
+        sfor<4>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            split<NP>(av[j], ta[j]);
+        });
+        sfor<5>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            split<NP>(bv[j], tb[j]);
+        });
+        sfor<NP>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            const uint2 pa = make_uint2(ta[0][k] | (ta[1][k] << 16), ta[2][k] | (ta[3][k] << 16));
+            const uint2 pe = make_uint2(tb[0][k] | (tb[1][k] << 16), tb[2][k] | (tb[3][k] << 16));
+            const uint2 po = make_uint2(tb[1][k] | (tb[2][k] << 16), tb[3][k] | (tb[4][k] << 16));
+            *reinterpret_cast<uint2*>(&lds_a[k][srow][scol]) = pa;
+            // copy 0: E[m] = b[m % 16]; copy 1: O[m] = b[(m + 1) % 16]
+            *reinterpret_cast<uint2*>(&lds_b[k][0][srow * 2 * W + scol]) = pe;
+            *reinterpret_cast<uint2*>(&lds_b[k][0][srow * 2 * W + scol + W]) = pe;
+            *reinterpret_cast<uint2*>(&lds_b[k][1][srow * 2 * W + scol]) = po;
+            *reinterpret_cast<uint2*>(&lds_b[k][1][srow * 2 * W + scol + W]) = po;
+        });
+    };
 
-        // ---- stage: lane holds row srow, columns scol .. scol + 3 ----
-        {
-            float av[4], bv[5];
-            sfor<4>([&](auto J) {
-                constexpr int j = decltype(J)::value;
-                av[j] = ra_[i % DEPTH].get(j);
-                bv[j] = rb_[i % DEPTH].get(j);
-            });
-            if constexpr (i + DEPTH < PER)
-                fetch(std::integral_constant<int, i + DEPTH>{}, ra_[i % DEPTH], rb_[i % DEPTH]);
-            bv[4] = from_quad_next(bv[0]);      // b[row][(scol + 4) % 16]
-            uint32_t ta[4][NP], tb[5][NP];
-            sfor<4>([&](auto J) {
-                constexpr int j = decltype(J)::value;
-                split<NP>(av[j], ta[j]);
-            });
-            sfor<5>([&](auto J) {
-                constexpr int j = decltype(J)::value;
-                split<NP>(bv[j], tb[j]);
-            });
-            sfor<NP>([&](auto K) {
-                constexpr int k = decltype(K)::value;
-                const uint2 pa = make_uint2(ta[0][k] | (ta[1][k] << 16), ta[2][k] | (ta[3][k] << 16));
-                const uint2 pe = make_uint2(tb[0][k] | (tb[1][k] << 16), tb[2][k] | (tb[3][k] << 16));
-                const uint2 po = make_uint2(tb[1][k] | (tb[2][k] << 16), tb[3][k] | (tb[4][k] << 16));
-                *reinterpret_cast<uint2*>(&lds_a[k][srow][scol]) = pa;
-                // copy 0: E[m] = b[m % 16]; copy 1: O[m] = b[(m + 1) % 16]
-                *reinterpret_cast<uint2*>(&lds_b[k][0][srow * 2 * W + scol]) = pe;
-                *reinterpret_cast<uint2*>(&lds_b[k][0][srow * 2 * W + scol + W]) = pe;
-                *reinterpret_cast<uint2*>(&lds_b[k][1][srow * 2 * W + scol]) = po;
-                *reinterpret_cast<uint2*>(&lds_b[k][1][srow * 2 * W + scol + W]) = po;
-            });
-        }
-        wave_sync();
-
-        // ---- C = P Q on MFMA: 8 k-chunks of 32 ----
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    // ---- C = P Q on MFMA: 8 k-chunks of 32, one accumulator per cross
+    // product x_i y_j (u16: all four; f32: i + j <= 2), and for u8 one per
+    // chunk parity, so the dependent MFMA chains are short ----
+    auto gemm = [&]() {
+        constexpr int NACC = NP == 1 ? 2 : (NP == 2 ? 4 : 6);
+        floatx4 part[NACC];
+        sfor<NACC>([&](auto Q) { part[decltype(Q)::value] = floatx4{0.f, 0.f, 0.f, 0.f}; });
         sfor<8>([&](auto Cc) {
             constexpr int c = decltype(Cc)::value;
             const int ra = (2 * c + (g >> 1) - i16) & (W - 1);
@@ -223,25 +226,39 @@ __global__ __launch_bounds__(256) void corr_direct16_kernel(CorrParams p)
                 uint32_t w4[4] = {q[0], q[1], q[2], q[3]};
                 B[k] = __builtin_bit_cast(bf16x8, w4);
             });
-            // cross products x_i y_j (u16: all four; f32: i + j <= 2), smallest first
             if constexpr (NP == 1) {
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[0], acc, 0, 0, 0);
+                part[c & 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[0], part[c & 1], 0, 0, 0);
             } else if constexpr (NP == 2) {
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[1], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[0], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[1], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[0], acc, 0, 0, 0);
+                part[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[1], part[0], 0, 0, 0);
+                part[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[0], part[1], 0, 0, 0);
+                part[2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[1], part[2], 0, 0, 0);
+                part[3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[0], part[3], 0, 0, 0);
             } else {
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[2], B[0], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[1], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[2], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[0], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[1], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[0], acc, 0, 0, 0);
+                part[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[2], B[0], part[0], 0, 0, 0);
+                part[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[1], part[1], 0, 0, 0);
+                part[2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[2], part[2], 0, 0, 0);
+                part[3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[0], part[3], 0, 0, 0);
+                part[4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[1], part[4], 0, 0, 0);
+                part[5] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[0], part[5], 0, 0, 0);
             }
         });
+        // smallest products first
+        floatx4 acc = part[0];
+        sfor<NACC - 1>([&](auto Q) {
+            constexpr int q = decltype(Q)::value + 1;
+            sfor<4>([&](auto R) {
+                constexpr int r = decltype(R)::value;
+                acc[r] = acc[r] + part[q][r];
+            });
+        });
+        return acc;
+    };
 
-        accs[i] = acc;
+    sfor<PER>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        stage(I);
+        wave_sync();
+        accs[i] = gemm();
         wave_sync();                            // staging buffers free for the next window
     });
 

@@ -160,10 +160,11 @@ def _check(rc: int) -> None:
         raise OpivError(rc, msg)
 
 
-def direct_chosen(win: int) -> bool:
-    """Whether OPIV_AUTO picks the direct MFMA path for this window size
-    (it does where that path exists: 16 x 16)."""
-    return win == 16
+def direct_chosen(win: int, dtype=np.float32) -> bool:
+    """Whether OPIV_AUTO picks the direct MFMA path.  It does not: measured
+    on MI355X the direct kernel at best ties the FFT kernel (8-bit frames at
+    16 x 16) and is slower for 16-bit and float frames."""
+    return False
 
 
 def version() -> str:
