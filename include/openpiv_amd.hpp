@@ -165,11 +165,76 @@ public:
         return out;
     }
 
+    /// gradients of sum_i cot[2i] u_i + cot[2i+1] v_i with respect to both
+    /// frames (pitch * img_h floats each); fwd = run()'s results for the same
+    /// frames, or nullptr to run the forward pass inside
+    void vjp(const void* a, const void* b, const opiv_window_result* fwd, const double* cot,
+             float* grad_a, float* grad_b) const
+    {
+        std::lock_guard<std::mutex> lock(ctx_.mutex());
+        check(opiv_vjp_pair(h_, a, b, fwd, cot, grad_a, grad_b));
+    }
+
+    /// (var u_i, var v_i) per window under i.i.d. pixel noise sigma
+    std::vector<double> variance(const void* a, const void* b, const opiv_window_result* fwd, double sigma) const
+    {
+        std::vector<double> var(2 * window_count());
+        std::lock_guard<std::mutex> lock(ctx_.mutex());
+        check(opiv_variance_pair(h_, a, b, fwd, sigma, var.data()));
+        return var;
+    }
+
     opiv_plan* get() const { return h_; }
 
 private:
     Context& ctx_;
     opiv_plan* h_ = nullptr;
+};
+
+/// Many frame pairs (device memory) x up to four window passes, replayed as
+/// one graph launch per run().
+class Batch {
+public:
+    Batch(Context& ctx, const opiv_batch_desc& desc, const std::vector<opiv_pair_desc>& pairs)
+        : ctx_(ctx), npasses_(desc.npasses), npairs_(pairs.size())
+    {
+        check(opiv_batch_create(ctx.get(), &desc, pairs.data(), pairs.size(), &h_));
+    }
+    ~Batch() { opiv_batch_destroy(h_); }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+
+    size_t window_count() const
+    {
+        size_t n = 0;
+        check(opiv_batch_window_count(h_, &n));
+        return n;
+    }
+
+    /// results of (pass p, pair i) start at offsets()[p * npairs + i]
+    std::vector<size_t> offsets() const
+    {
+        std::vector<size_t> off(npasses_ * npairs_ + 1);
+        check(opiv_batch_offsets(h_, off.data()));
+        return off;
+    }
+
+    /// new device frames of the same sizes for the next run()
+    void set_pairs(const std::vector<opiv_pair_desc>& pairs) { check(opiv_batch_set_pairs(h_, pairs.data(), pairs.size())); }
+
+    /// enqueue every pass on the context stream; d_out: window_count() results in device memory
+    void run(opiv_window_result* d_out, size_t n)
+    {
+        std::lock_guard<std::mutex> lock(ctx_.mutex());
+        check(opiv_batch_run(h_, d_out, n));
+    }
+
+    opiv_batch* get() const { return h_; }
+
+private:
+    Context& ctx_;
+    size_t npasses_, npairs_;
+    opiv_batch* h_ = nullptr;
 };
 
 } // namespace openpiv_amd

@@ -11,9 +11,12 @@ PROCESS_BIN = ROOT / "openpiv-cxx_amd" / "lib" / "opiv_process"
 def build_dropin_check(tmp: Path) -> Path:
     exe = tmp / "dropin_check"
     lib = ROOT / "openpiv-cxx_amd" / "lib"
-    subprocess.run(["g++", "-std=c++17", "-O1", "-I", str(ROOT / "include"),
+    rocm = Path("/opt/rocm")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", "-I", str(ROOT / "include"),
+                    "-I", str(rocm / "include"),
                     str(ROOT / "tests" / "cpp" / "dropin_check.cpp"), "-o", str(exe),
-                    f"-L{lib}", "-lopenpiv_amd", f"-Wl,-rpath,{lib}"], check=True)
+                    f"-L{lib}", "-lopenpiv_amd", f"-Wl,-rpath,{lib}",
+                    f"-L{rocm / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{rocm / 'lib'}"], check=True)
     return exe
 
 

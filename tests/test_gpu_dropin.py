@@ -84,3 +84,39 @@ def test_process_16bit_limit_search_and_overlap(tmp_path):
         assert r.returncode == 0, r.stderr
         ref = orc.process_pair(a, b, 32, off, off, limit_search=limit, threads=8)
         check_output(r.stdout, [ref[k] for k in ("x", "y", "u", "v", "sn")], (ref["flags"] & 1) != 0)
+
+
+def test_cpp_plan_adjoint_matches_bindings(dropin, tmp_path, gpu_ctx):
+    """Plan::vjp / Plan::variance of the C++ layer give the C ABI's results
+    (the same calls the ctypes bindings make)."""
+    w = h = 256
+    a, b = oa.synth_pair(w, h, seed=12, params=(1.6, -0.8), dtype=np.float32)
+    plan = oa.Plan(gpu_ctx, w, h, 32, dtype=np.float32)
+    cot = np.random.default_rng(3).normal(size=(plan.count, 2))
+    for name, arr in (("a", a), ("b", b)):
+        arr.tofile(tmp_path / f"{name}.bin")
+    cot.tofile(tmp_path / "cot.bin")
+    subprocess.run([str(dropin), "adjoint", str(w), str(h), "32", str(tmp_path / "a.bin"), str(tmp_path / "b.bin"),
+                    str(tmp_path / "cot.bin"), str(tmp_path / "o.bin")], check=True, timeout=60)
+    raw = (tmp_path / "o.bin").read_bytes()
+    ga = np.frombuffer(raw[:w * h * 4], np.float32).reshape(h, w)
+    gb = np.frombuffer(raw[w * h * 4:2 * w * h * 4], np.float32).reshape(h, w)
+    var = np.frombuffer(raw[2 * w * h * 4:], np.float64).reshape(-1, 2)
+    fwd = plan.run(a, b)
+    ra, rb = plan.vjp(a, b, cot, fwd)
+    assert np.array_equal(ga, ra) and np.array_equal(gb, rb)
+    assert np.array_equal(var, plan.variance(a, b, 2.0, fwd))
+
+
+def test_cpp_batch_matches_single_pair(dropin, tmp_path, gpu_ctx):
+    """Batch of the C++ layer (one pair, passes 32 and 16) against single-pair
+    plans: bit-identical records."""
+    w, h = 320, 256
+    a, b = oa.synth_pair(w, h, seed=13, params=(-1.2, 2.1), dtype=np.float32)
+    a.tofile(tmp_path / "a.bin")
+    b.tofile(tmp_path / "b.bin")
+    subprocess.run([str(dropin), "batch", str(w), str(h), str(tmp_path / "a.bin"), str(tmp_path / "b.bin"),
+                    str(tmp_path / "o.bin")], check=True, timeout=60)
+    got = np.frombuffer((tmp_path / "o.bin").read_bytes(), oa.RESULT_DTYPE)
+    ref = np.concatenate([oa.Plan(gpu_ctx, w, h, win, dtype=np.float32).run(a, b) for win in (32, 16)])
+    assert got.tobytes() == ref.tobytes()
