@@ -151,6 +151,18 @@ int opiv_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t varian
 int opiv_auto_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n,
                                 const double* a, double* planes);
 
+/* algos::direction (fft_common.h:9-12) */
+typedef enum opiv_direction {
+    OPIV_FORWARD = 0,        /* exp(-i 2 pi k n / N) */
+    OPIV_REVERSE = 1         /* exp(+i 2 pi k n / N), un-normalised */
+} opiv_direction;
+
+/* FFT::transform (fft.h:86-119): un-normalised 2-D DFT of n stacked
+ * win*win complex windows (interleaved re, im FP64, row-major), computed in
+ * FP32.  in == out is allowed.  Host or device pointers. */
+int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t direction,
+                               const double* in, double* out);
+
 /* ---- adjoint: displacement sensitivities (no reference counterpart) ----
  *
  * Derivatives of each window's vector (u, v) of main.cpp:227 with respect to

@@ -19,7 +19,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <complex>
 #include <mutex>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -96,6 +98,24 @@ public:
         return correlate(a, static_cast<const ImageT*>(nullptr), OPIV_CIRCULAR_FFT_FP32);
     }
 
+    /// fft.h:86-119: un-normalised 2-D DFT (FP32 arithmetic); real or complex
+    /// pixels in, complex values out in row-major order
+    template <class ImageT>
+    std::vector<std::complex<double>> transform(const ImageT& in, opiv_direction d = OPIV_FORWARD) const
+    {
+        if (in.width() != w_ || in.height() != h_)
+            throw error(OPIV_E_SIZE, "image size is different from expected: [" + std::to_string(in.width()) + "," +
+                                         std::to_string(in.height()) + "], [" + std::to_string(w_) + "," +
+                                         std::to_string(h_) + "]");
+        const size_t n = static_cast<size_t>(w_) * h_;
+        std::vector<std::complex<double>> z(n), out(n);
+        for (size_t i = 0; i < n; ++i) z[i] = to_complex(in[i]);
+        std::lock_guard<std::mutex> lock(ctx_.mutex());
+        check(opiv_fft_transform_windows(ctx_.get(), w_, 1, d, reinterpret_cast<const double*>(z.data()),
+                                         reinterpret_cast<double*>(out.data())));
+        return out;
+    }
+
     /// Batched form: n stacked w*h windows (host or device memory).
     void cross_correlate(const double* a, const double* b, size_t n, double* planes,
                          opiv_variant variant = OPIV_CIRCULAR_FFT_FP32) const
@@ -105,6 +125,23 @@ public:
     }
 
 private:
+    // a real pixel (anything convertible to double, e.g. core::g<T>), a
+    // std::complex, or a pixel with real/imag members (core::complex<T>,
+    // pixel_types.h:200-201)
+    template <class T, class = void>
+    struct has_real_member : std::false_type {};
+    template <class T>
+    struct has_real_member<T, std::void_t<decltype(std::declval<const T&>().real + std::declval<const T&>().imag)>>
+        : std::true_type {};
+
+    template <class T>
+    static std::complex<double> to_complex(const T& v)
+    {
+        if constexpr (std::is_convertible_v<T, double>) return { static_cast<double>(v), 0.0 };
+        else if constexpr (has_real_member<T>::value) return { static_cast<double>(v.real), static_cast<double>(v.imag) };
+        else return { static_cast<double>(v.real()), static_cast<double>(v.imag()) };
+    }
+
     template <class ImageT>
     ImageT correlate(const ImageT& a, const ImageT* b, opiv_variant variant) const
     {

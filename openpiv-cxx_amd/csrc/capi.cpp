@@ -15,6 +15,7 @@
 #include "corr_fft.hpp"
 #include "ctx.hpp"
 #include "openpiv_amd.h"
+#include "transform.hpp"
 
 using namespace opiv::detail;
 
@@ -452,6 +453,36 @@ int opiv_auto_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n, const dou
 {
     // FFT(a) conj(FFT(a)) = |FFT(a)|^2: the cross-correlation of a with itself
     return correlate_stacked(ctx, win, n, OPIV_CIRCULAR_FFT_FP32, a, a, planes);
+}
+
+int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t direction,
+                               const double* in, double* out)
+{
+    if (!ctx || !in || !out) return fail(OPIV_E_INVALID, "transform: null argument");
+    if (!is_pow2(win)) return fail(OPIV_E_NOT_POW2, "dimensions must be power of 2: [%u,%u]", win, win);
+    if (win < 8 || win > 64) return fail(OPIV_E_UNSUPPORTED, "window %u outside the supported range [8, 64]", win);
+    if (direction != OPIV_FORWARD && direction != OPIV_REVERSE)
+        return fail(OPIV_E_INVALID, "unknown direction %d", direction);
+    if (n == 0) return OPIV_OK;
+    if (n > 0x7fffffffull / win) return fail(OPIV_E_UNSUPPORTED, "too many windows");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t bytes = n * win * win * 2 * sizeof(double);
+    const double* din = in;
+    if (!is_device_ptr(in)) {
+        HIP_TRY(ctx->stage_a.ensure(bytes));
+        HIP_TRY(hipMemcpyAsync(ctx->stage_a.p, in, bytes, hipMemcpyHostToDevice, ctx->stream));
+        din = (const double*)ctx->stage_a.p;
+    }
+    const bool out_dev = is_device_ptr(out);
+    double* dout = out;
+    if (!out_dev) {
+        HIP_TRY(ctx->stage_out.ensure(bytes));
+        dout = (double*)ctx->stage_out.p;
+    }
+    HIP_TRY(opiv::launch_fft_transform(din, dout, (int)win, n, direction == OPIV_FORWARD, ctx->stream));
+    if (!out_dev) HIP_TRY(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return OPIV_OK;
 }
 
 } // extern "C"

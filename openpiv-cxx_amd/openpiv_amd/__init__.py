@@ -34,6 +34,8 @@ REAL_COMPAT = 1
 DIRECT_BF16_SPLIT = 2
 AUTO = 3
 # opiv_flag
+FORWARD = 0
+REVERSE = 1
 FLAG_FEWER_THAN_2_PEAKS = 1
 FLAG_NONFINITE_FIT = 2
 FLAG_NEAR_TIE = 4
@@ -53,7 +55,7 @@ EXPORTS = (
     "opiv_grid_offsets", "opiv_grid",
     "opiv_plan_create", "opiv_plan_destroy", "opiv_plan_window_count", "opiv_plan_grid",
     "opiv_run_pair", "opiv_run_pair_async",
-    "opiv_correlate_windows", "opiv_auto_correlate_windows",
+    "opiv_correlate_windows", "opiv_auto_correlate_windows", "opiv_fft_transform_windows",
     "opiv_batch_create", "opiv_batch_destroy", "opiv_batch_window_count", "opiv_batch_offsets",
     "opiv_batch_set_pairs", "opiv_batch_shape", "opiv_batch_run",
     "opiv_vjp_pair", "opiv_variance_pair",
@@ -137,6 +139,7 @@ def load() -> ctypes.CDLL:
     lib.opiv_run_pair_async.argtypes = [vp, vp, vp, vp, sz]
     lib.opiv_correlate_windows.argtypes = [vp, u32, sz, i32, vp, vp, vp]
     lib.opiv_auto_correlate_windows.argtypes = [vp, u32, sz, vp, vp]
+    lib.opiv_fft_transform_windows.argtypes = [vp, u32, sz, i32, vp, vp]
     lib.opiv_batch_create.argtypes = [vp, ctypes.POINTER(BatchDesc), ctypes.POINTER(PairDesc), sz,
                                       ctypes.POINTER(vp)]
     lib.opiv_batch_destroy.argtypes = [vp]
@@ -256,6 +259,18 @@ class Context:
         out = np.empty_like(a)
         _check(self._lib.opiv_auto_correlate_windows(self.handle, win, n, a.ctypes.data, out.ctypes.data))
         return out
+
+    def transform(self, z: np.ndarray, direction: int = FORWARD) -> np.ndarray:
+        """FFT::transform (fft.h:86-119) of one or n stacked square windows
+        (complex or real input); complex128 output."""
+        z = np.ascontiguousarray(z, dtype=np.complex128)
+        single = z.ndim == 2
+        if single:
+            z = z[None]
+        n, win = z.shape[0], z.shape[1]
+        out = np.empty_like(z)
+        _check(self._lib.opiv_fft_transform_windows(self.handle, win, n, direction, z.ctypes.data, out.ctypes.data))
+        return out[0] if single else out
 
 
 class Plan:

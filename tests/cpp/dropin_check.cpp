@@ -4,6 +4,8 @@
 //   dropin_check correlate WIN VARIANT A.bin B.bin OUT.bin   (FP64 windows)
 //   dropin_check pow2 W H        -> prints the exception text
 //   dropin_check nodevice        -> prints the error code of Context()
+//   dropin_check transform WIN forward|reverse IN.bin OUT.bin
+//       HipFFT::transform of one window of complex FP64 pixels
 //   dropin_check adjoint W H WIN A.bin B.bin COT.bin OUT.bin
 //       f32 frames, 50% overlap: Plan::vjp (grad_a, grad_b as f32) then
 //       Plan::variance (sigma 2, f64) written one after the other
@@ -28,6 +30,18 @@ struct gimage {   // stands in for core::gf_image's interface
     uint32_t height() const { return h; }
     double& operator[](size_t i) { return px[i]; }
     const double& operator[](size_t i) const { return px[i]; }
+};
+
+struct cimage {   // complex pixels with real/imag members, like core::complex<double>
+    struct px_t {
+        double real, imag;
+    };
+    uint32_t w, h;
+    std::vector<px_t> px;
+    cimage(uint32_t w_, uint32_t h_) : w(w_), h(h_), px((size_t)w_ * h_) {}
+    uint32_t width() const { return w; }
+    uint32_t height() const { return h; }
+    const px_t& operator[](size_t i) const { return px[i]; }
 };
 
 static gimage read_window(const char* path, uint32_t win)
@@ -68,6 +82,19 @@ int main(int argc, char** argv)
                                                : fft.cross_correlate(a, b);
             std::ofstream os(argv[6], std::ios::binary);
             os.write(reinterpret_cast<const char*>(c.px.data()), (std::streamsize)(c.px.size() * sizeof(double)));
+            return 0;
+        }
+        if (cmd == "transform" && argc >= 6) {
+            const uint32_t win = (uint32_t)std::stoul(argv[2]);
+            const bool fwd = std::string(argv[3]) == "forward";
+            cimage in(win, win);
+            std::ifstream(argv[4], std::ios::binary)
+                .read(reinterpret_cast<char*>(in.px.data()), (std::streamsize)(in.px.size() * 16));
+            openpiv_amd::Context ctx(0);
+            openpiv_amd::HipFFT fft(ctx, win, win);
+            const auto out = fft.transform(in, fwd ? OPIV_FORWARD : OPIV_REVERSE);
+            std::ofstream os(argv[5], std::ios::binary);
+            os.write(reinterpret_cast<const char*>(out.data()), (std::streamsize)(out.size() * 16));
             return 0;
         }
         if (cmd == "adjoint" && argc >= 9) {

@@ -120,3 +120,15 @@ def test_cpp_batch_matches_single_pair(dropin, tmp_path, gpu_ctx):
     got = np.frombuffer((tmp_path / "o.bin").read_bytes(), oa.RESULT_DTYPE)
     ref = np.concatenate([oa.Plan(gpu_ctx, w, h, win, dtype=np.float32).run(a, b) for win in (32, 16)])
     assert got.tobytes() == ref.tobytes()
+
+
+@pytest.mark.parametrize("direction", ["forward", "reverse"])
+def test_hipfft_transform_matches_oracle(dropin, tmp_path, direction):
+    rng = np.random.default_rng(7)
+    z = rng.normal(size=(32, 32)) + 1j * rng.normal(size=(32, 32))
+    z.astype(np.complex128).tofile(tmp_path / "z.bin")
+    subprocess.run([str(dropin), "transform", "32", direction, str(tmp_path / "z.bin"), str(tmp_path / "o.bin")],
+                   check=True, timeout=60)
+    got = np.fromfile(tmp_path / "o.bin", dtype=np.complex128).reshape(32, 32)
+    ref = orc.FFT(32).transform(z, forward=direction == "forward")
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()

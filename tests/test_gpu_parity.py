@@ -267,3 +267,28 @@ def test_errors(gpu_ctx):
     bad = np.zeros(3, oa.RESULT_DTYPE)
     rc = oa.load().opiv_run_pair(plan.handle, a.ctypes.data, a.ctypes.data, bad.ctypes.data, 3)
     assert rc == 3 and "differs" in oa.load().opiv_last_error().decode()
+
+
+@pytest.mark.parametrize("win", [8, 16, 32, 64])
+def test_transform_matches_oracle(gpu_ctx, win):
+    """FFT::transform (fft.h:86-119) of stacked complex windows, both
+    directions, against the FP64 oracle; plus the DFT of a single complex
+    exponential, which is W^2 at its frequency and 0 elsewhere."""
+    rng = np.random.default_rng(win)
+    z = rng.normal(size=(5, win, win)) + 1j * rng.normal(size=(5, win, win))
+    f = orc.FFT(win)
+    for direction, fwd in ((oa.FORWARD, True), (oa.REVERSE, False)):
+        got = gpu_ctx.transform(z, direction)
+        for i in range(len(z)):
+            ref = f.transform(z[i], forward=fwd)
+            assert np.abs(got[i] - ref).max() <= 1e-5 * np.abs(ref).max(), (direction, i)
+    ky, kx = 3 % win, 5 % win
+    y, x = np.mgrid[0:win, 0:win]
+    tone = np.exp(2j * np.pi * (ky * y + kx * x) / win)
+    spec = gpu_ctx.transform(tone, oa.FORWARD)
+    expect = np.zeros((win, win), complex)
+    expect[ky, kx] = win * win
+    assert np.abs(spec - expect).max() <= 1e-4 * win * win
+    # real input: the transform of the real part is Hermitian
+    r = gpu_ctx.transform(z[0].real, oa.FORWARD)
+    assert np.abs(r - np.conj(r[(-np.arange(win)) % win][:, (-np.arange(win)) % win])).max() <= 1e-5 * np.abs(r).max()
