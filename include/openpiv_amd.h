@@ -163,6 +163,21 @@ typedef enum opiv_direction {
 int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t direction,
                                const double* in, double* out);
 
+/* ---- background subtraction: examples/average_subtract/main.cpp:98-159 ----
+ *
+ * For n >= 2 frames of one size and pixel type (frames[i]: host or device,
+ * pitch elements per row, 0 = img_w): mean = (frame 1 + ... + frame n-1) /
+ * (n - 1) -- the reference sizes its accumulator from the first input and
+ * averages every later one (main.cpp:98-131) -- and for every frame i the
+ * 16-bit image the reference's PNM writer saves for frame_i - mean
+ * (pnm_image_loader.cpp:367-392: (uint16)(65535 * ((d - min) / range)),
+ * range = max - min, or 1 when equal).  out[i]: img_w*img_h uint16,
+ * row-major, host or device; mean: img_w*img_h doubles or NULL.  FP64
+ * arithmetic in the reference's order: bit-identical outputs.  Returns when
+ * the outputs are written. */
+int opiv_average_subtract(opiv_ctx* ctx, const void* const* frames, size_t n, uint32_t img_w, uint32_t img_h,
+                          size_t pitch, int32_t dtype, uint16_t* const* out, double* mean);
+
 /* ---- adjoint: displacement sensitivities (no reference counterpart) ----
  *
  * Derivatives of each window's vector (u, v) of main.cpp:227 with respect to

@@ -9,8 +9,10 @@
 
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "adjoint.hpp"
+#include "background.hpp"
 #include "corr_direct.hpp"
 #include "corr_fft.hpp"
 #include "ctx.hpp"
@@ -61,7 +63,8 @@ int opiv_ctx_destroy(opiv_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto* b : { &ctx->stage_a, &ctx->stage_b, &ctx->stage_out, &ctx->stage_fwd, &ctx->stage_cot,
-                     &ctx->stage_ga, &ctx->stage_gb, &ctx->stage_var, &ctx->adj_coef })
+                     &ctx->stage_ga, &ctx->stage_gb, &ctx->stage_var, &ctx->adj_coef, &ctx->bg_frames,
+                     &ctx->bg_out, &ctx->bg_mean, &ctx->bg_table, &ctx->bg_range })
         b->release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -453,6 +456,79 @@ int opiv_auto_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n, const dou
 {
     // FFT(a) conj(FFT(a)) = |FFT(a)|^2: the cross-correlation of a with itself
     return correlate_stacked(ctx, win, n, OPIV_CIRCULAR_FFT_FP32, a, a, planes);
+}
+
+int opiv_average_subtract(opiv_ctx* ctx, const void* const* frames, size_t n, uint32_t img_w, uint32_t img_h,
+                          size_t pitch, int32_t dtype, uint16_t* const* out, double* mean)
+{
+    if (!ctx || !frames || !out) return fail(OPIV_E_INVALID, "average_subtract: null argument");
+    // examples/average_subtract/main.cpp:85-89
+    if (n < 2) return fail(OPIV_E_INVALID, "require two or more input images");
+    if (n > 0x7fffffff) return fail(OPIV_E_UNSUPPORTED, "too many frames");
+    if (img_w == 0 || img_h == 0) return fail(OPIV_E_INVALID, "average_subtract: empty frame");
+    const size_t esz = dtype_size(dtype);
+    if (esz == 0) return fail(OPIV_E_INVALID, "unknown dtype %d", dtype);
+    if (pitch == 0) pitch = img_w;
+    if (pitch < img_w) return fail(OPIV_E_INVALID, "pitch %zu smaller than width %u", pitch, img_w);
+    for (size_t i = 0; i < n; ++i)
+        if (!frames[i] || !out[i]) return fail(OPIV_E_INVALID, "average_subtract: null frame or output %zu", i);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t fbytes = pitch * img_h * esz, obytes = (size_t)img_w * img_h * sizeof(uint16_t);
+    const size_t mbytes = (size_t)img_w * img_h * sizeof(double);
+
+    // device frame and output pointers; host ones are staged
+    std::vector<const void*> dframes(n);
+    std::vector<uint16_t*> douts(n);
+    std::vector<bool> host_out(n);
+    size_t nhost_in = 0, nhost_out = 0;
+    for (size_t i = 0; i < n; ++i) {
+        nhost_in += is_device_ptr(frames[i]) ? 0 : 1;
+        host_out[i] = !is_device_ptr(out[i]);
+        nhost_out += host_out[i] ? 1 : 0;
+    }
+    if (nhost_in) HIP_TRY(ctx->bg_frames.ensure(nhost_in * fbytes));
+    if (nhost_out) HIP_TRY(ctx->bg_out.ensure(nhost_out * obytes));
+    for (size_t i = 0, hi = 0, ho = 0; i < n; ++i) {
+        if (is_device_ptr(frames[i])) {
+            dframes[i] = frames[i];
+        } else {
+            void* dst = static_cast<char*>(ctx->bg_frames.p) + (hi++) * fbytes;
+            HIP_TRY(hipMemcpyAsync(dst, frames[i], fbytes, hipMemcpyHostToDevice, ctx->stream));
+            dframes[i] = dst;
+        }
+        douts[i] = host_out[i] ? reinterpret_cast<uint16_t*>(static_cast<char*>(ctx->bg_out.p) + (ho++) * obytes)
+                               : out[i];
+    }
+    const bool mean_dev = mean && is_device_ptr(mean);
+    double* dmean = mean_dev ? mean : nullptr;
+    if (!dmean) {
+        HIP_TRY(ctx->bg_mean.ensure(mbytes));
+        dmean = static_cast<double*>(ctx->bg_mean.p);
+    }
+    HIP_TRY(ctx->bg_table.ensure(2 * n * sizeof(void*)));
+    HIP_TRY(ctx->bg_range.ensure(2 * n * sizeof(unsigned long long)));
+    std::vector<const void*> table(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+        table[i] = dframes[i];
+        table[n + i] = douts[i];
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->bg_table.p, table.data(), 2 * n * sizeof(void*), hipMemcpyHostToDevice, ctx->stream));
+
+    opiv::BgParams p{};
+    p.frames = static_cast<const void* const*>(ctx->bg_table.p);
+    p.out = reinterpret_cast<uint16_t* const*>(static_cast<char*>(ctx->bg_table.p) + n * sizeof(void*));
+    p.n = (int)n;
+    p.w = (int)img_w;
+    p.h = (int)img_h;
+    p.pitch = pitch;
+    p.mean = dmean;
+    p.range = static_cast<unsigned long long*>(ctx->bg_range.p);
+    HIP_TRY(opiv::launch_average_subtract(p, dtype, ctx->stream));
+    for (size_t i = 0; i < n; ++i)
+        if (host_out[i]) HIP_TRY(hipMemcpyAsync(out[i], douts[i], obytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (mean && !mean_dev) HIP_TRY(hipMemcpyAsync(mean, dmean, mbytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return OPIV_OK;
 }
 
 int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t direction,

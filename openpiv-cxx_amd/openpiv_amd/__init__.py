@@ -59,6 +59,7 @@ EXPORTS = (
     "opiv_batch_create", "opiv_batch_destroy", "opiv_batch_window_count", "opiv_batch_offsets",
     "opiv_batch_set_pairs", "opiv_batch_shape", "opiv_batch_run",
     "opiv_vjp_pair", "opiv_variance_pair",
+    "opiv_average_subtract",
 )
 
 
@@ -150,6 +151,7 @@ def load() -> ctypes.CDLL:
     lib.opiv_batch_run.argtypes = [vp, vp, sz]
     lib.opiv_vjp_pair.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.opiv_variance_pair.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp]
+    lib.opiv_average_subtract.argtypes = [vp, vp, sz, u32, u32, sz, i32, vp, vp]
     for name in EXPORTS:
         if name not in ("opiv_last_error", "opiv_version", "opiv_ctx_stream"):
             getattr(lib, name).restype = ctypes.c_int
@@ -259,6 +261,29 @@ class Context:
         out = np.empty_like(a)
         _check(self._lib.opiv_auto_correlate_windows(self.handle, win, n, a.ctypes.data, out.ctypes.data))
         return out
+
+    def average_subtract(self, frames, *, return_mean: bool = False):
+        """examples/average_subtract/main.cpp:98-159 on a stack of n >= 2 host
+        frames (n, h, w) of uint8/uint16/float32/float64: the uint16 images
+        the reference writes (and the mean of frames 1..n-1)."""
+        frames = np.ascontiguousarray(frames)
+        n, h, w = frames.shape
+        dt = DTYPE_CODES[frames.dtype]
+        out = np.zeros((n, h, w), np.uint16)
+        mean = np.zeros((h, w), np.float64)
+        fp = (ctypes.c_void_p * n)(*[frames[i].ctypes.data for i in range(n)])
+        op = (ctypes.c_void_p * n)(*[out[i].ctypes.data for i in range(n)])
+        _check(self._lib.opiv_average_subtract(self.handle, fp, n, w, h, 0, dt, op, mean.ctypes.data))
+        return (out, mean) if return_mean else out
+
+    def average_subtract_device(self, frame_ptrs, out_ptrs, w: int, h: int, dtype, pitch: int = 0,
+                                mean_ptr: int | None = None) -> None:
+        """The same on device frames and outputs (lists of pointers)."""
+        n = len(frame_ptrs)
+        fp = (ctypes.c_void_p * n)(*frame_ptrs)
+        op = (ctypes.c_void_p * n)(*out_ptrs)
+        _check(self._lib.opiv_average_subtract(self.handle, fp, n, w, h, pitch, DTYPE_CODES[np.dtype(dtype)],
+                                               op, mean_ptr))
 
     def transform(self, z: np.ndarray, direction: int = FORWARD) -> np.ndarray:
         """FFT::transform (fft.h:86-119) of one or n stacked square windows

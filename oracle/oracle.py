@@ -83,6 +83,7 @@ def load():
         lib.oref_vjp_pair.argtypes = [ctypes.POINTER(Job), vp, vp, vp]
         lib.oref_vjp_pair.restype = ctypes.c_long
         lib.oref_variance_pair.argtypes = [ctypes.POINTER(Job), ctypes.c_double, vp]
+        lib.oref_average_subtract.argtypes = [vp, ctypes.c_size_t, u32, u32, vp, vp]
         lib.oref_variance_pair.restype = ctypes.c_long
         lib.oref_process_window.argtypes = [vp, vp, vp, u32, ctypes.c_int32, ctypes.c_int32, u32, ctypes.c_int,
                                             ctypes.c_int, vp, vp]
@@ -267,3 +268,25 @@ def variance_pair(a, b, win, off_x, off_y, sigma, limit_search=False):
     if load().oref_variance_pair(ctypes.byref(job), sigma, var.ctypes.data) < 0:
         raise ValueError("oref_variance_pair failed")
     return var
+
+
+# ---------------------------------------------------------------- background subtraction
+
+def average_subtract(frames):
+    """examples/average_subtract/main.cpp:98-159 on n stacked frames (any
+    numeric dtype, values as loaded): (mean f64 (h, w), out uint16 (n, h, w))."""
+    f = np.ascontiguousarray(frames, dtype=np.float64)
+    n, h, w = f.shape
+    mean = np.zeros((h, w))
+    out = np.zeros((n, h, w), np.uint16)
+    if load().oref_average_subtract(f.ctypes.data, n, w, h, mean.ctypes.data, out.ctypes.data) != 0:
+        raise ValueError("average_subtract needs two or more frames")
+    return mean, out
+
+
+def reference_average_subtract(pgm_paths, out_prefix):
+    """The reference's own pipeline (loader, image arithmetic, find_image_range,
+    PNM writer) through oracle/_ref/ref_piv avgsub: (mean, [PNM bytes per frame])."""
+    run_reference(["avgsub", out_prefix] + [str(p) for p in pgm_paths])
+    mean = np.fromfile(f"{out_prefix}mean.bin", dtype=np.float64)
+    return mean, [Path(f"{out_prefix}{i}.pnm").read_bytes() for i in range(len(pgm_paths))]

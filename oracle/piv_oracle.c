@@ -710,3 +710,42 @@ long oref_variance_pair(const oref_job* job, double sigma, double* var)
 {
     return adjoint_pair(job, NULL, NULL, NULL, sigma, var);
 }
+
+
+/* ---- background subtraction (examples/average_subtract/main.cpp:98-159) ---- */
+
+int oref_average_subtract(const double* frames, size_t n, uint32_t w, uint32_t h, double* mean, uint16_t* out)
+{
+    if (n < 2) return -1;
+    const size_t np = (size_t)w * h;
+    /* main.cpp:98-122: avg starts at zero, frames 1..n-1 added in order */
+    for (size_t p = 0; p < np; ++p) mean[p] = 0.0;
+    size_t processed = 0;
+    for (size_t i = 1; i < n; ++i) {
+        const double* f = frames + i * np;
+        for (size_t p = 0; p < np; ++p) mean[p] = mean[p] + f[p];
+        ++processed;
+    }
+    /* main.cpp:131: avg / g_f{processed} */
+    const double div = (double)processed;
+    for (size_t p = 0; p < np; ++p) mean[p] = mean[p] / div;
+    double* d = (double*)malloc(np * sizeof(double));
+    if (!d) return -1;
+    for (size_t i = 0; i < n; ++i) {
+        const double* f = frames + i * np;
+        for (size_t p = 0; p < np; ++p) d[p] = f[p] - mean[p];          /* main.cpp:146 */
+        /* stats.h:22-31 */
+        double mn = d[0], mx = d[0];
+        for (size_t p = 1; p < np; ++p) {
+            mn = d[p] < mn ? d[p] : mn;
+            mx = d[p] > mx ? d[p] : mx;
+        }
+        /* pnm_image_loader.cpp:369-391 */
+        double range = mx - mn;
+        if (mx == mn) range = 1.0;
+        uint16_t* o = out + i * np;
+        for (size_t p = 0; p < np; ++p) o[p] = (uint16_t)(65535.0 * ((d[p] - mn) / range));
+    }
+    free(d);
+    return 0;
+}

@@ -160,6 +160,18 @@ long oref_vjp_pair(const oref_job* job, const double* cot, double* grad_a, doubl
  * (0 for windows without a valid fit) */
 long oref_variance_pair(const oref_job* job, double sigma, double* var);
 
+/* ---- background subtraction: examples/average_subtract/main.cpp:98-159 ----
+ * frames: n row-major w*h images (pixel values as loaded, FP64).
+ * mean = (frames[1] + ... + frames[n-1]) / (n - 1): the reference sizes its
+ * accumulator from the first input and adds every later one, in input order
+ * (main.cpp:98-122).  For every frame i, d = frames[i] - mean (main.cpp:146)
+ * and out[i] = the 16-bit image the PNM writer saves for d
+ * (pnm_image_loader.cpp:367-392): (uint16)(65535 * ((d - min) / range)),
+ * min/max by algos::find_image_range (stats.h:15-34), range = max - min, or 1
+ * when max == min.  Returns -1 when n < 2 (the reference's "failed to load
+ * any files"), else 0. */
+int oref_average_subtract(const double* frames, size_t n, uint32_t w, uint32_t h, double* mean, uint16_t* out);
+
 #ifdef __cplusplus
 }
 #endif

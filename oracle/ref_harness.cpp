@@ -21,6 +21,10 @@
 //   ref_piv gridf IMG_W IMG_H WIN_W WIN_H FRACTION
 //   ref_piv peaks W H NUM RADIUS IN.bin          (prints x y value per peak)
 //   ref_piv time A.bin B.bin W H WIN OFF_X OFF_Y THREADS REPS (prints us per rep)
+//   ref_piv avgsub OUT_PREFIX F0.pgm F1.pgm ...
+//       examples/average_subtract/main.cpp:98-159 on PNM frames loaded by the
+//       reference's loader: writes OUT_PREFIX<i>.pnm with the reference's PNM
+//       writer and OUT_PREFIXmean.bin (the average, f64)
 
 #include <chrono>
 #include <cstdio>
@@ -31,10 +35,12 @@
 #include <vector>
 
 #include "algos/fft.h"
+#include "algos/stats.h"
 #include "core/grid.h"
 #include "core/image.h"
 #include "core/image_utils.h"
 #include "core/vector.h"
+#include "loaders/image_loader.h"
 #include "threadpool.hpp"
 
 using namespace openpiv;
@@ -272,6 +278,57 @@ int cmd_time(int argc, char** argv)
     return 0;
 }
 
+// load_from_file of examples/average_subtract/main.cpp:21-42 (the error
+// path returns an empty image, as there)
+core::gf_image load_pnm(const std::string& filename)
+{
+    std::ifstream is(filename, std::ios::binary);
+    core::gf_image image;
+    if (!is.is_open()) return image;
+    auto loader{ core::image_loader_registry::find(is) };
+    if (!loader) return image;
+    loader->load(is, image);
+    return image;
+}
+
+// examples/average_subtract/main.cpp:98-159: the average of every input but
+// the first (the first only sets the size), subtracted from every input, each
+// result written by the PNM writer
+int cmd_avgsub(int argc, char** argv)
+{
+    if (argc < 5) return 1;
+    const std::string prefix = argv[2];
+    std::vector<std::string> inputs(argv + 3, argv + argc);
+    core::gf_image avg;
+    {
+        auto image = load_pnm(inputs[0]);
+        avg.resize(image.size());
+    }
+    size_t processed = 0;
+    for (size_t i = 1; i < inputs.size(); ++i) {
+        auto image = load_pnm(inputs[i]);
+        avg = avg + image;
+        ++processed;
+    }
+    if (processed == 0) return 1;
+    avg = avg / core::g_f{ processed };
+    {
+        std::ofstream os(prefix + "mean.bin", std::ios::binary);
+        for (uint32_t h = 0; h < avg.height(); ++h)
+            os.write(reinterpret_cast<const char*>(avg.line(h)), (std::streamsize)(avg.width() * sizeof(double)));
+    }
+    std::shared_ptr<core::image_loader> writer{ core::image_loader_registry::find("image/x-portable-anymap") };
+    if (!writer) return 1;
+    for (size_t i = 0; i < inputs.size(); ++i) {
+        auto image = load_pnm(inputs[i]);
+        image = image - avg;
+        std::fstream os(prefix + std::to_string(i) + ".pnm",
+                        std::ios_base::trunc | std::ios_base::out | std::ios_base::binary);
+        writer->save(os, image);
+    }
+    return 0;
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -285,5 +342,6 @@ int main(int argc, char** argv)
     if (cmd == "gridf") return cmd_grid(argc, argv, true);
     if (cmd == "peaks") return cmd_peaks(argc, argv);
     if (cmd == "time") return cmd_time(argc, argv);
+    if (cmd == "avgsub") return cmd_avgsub(argc, argv);
     return 1;
 }

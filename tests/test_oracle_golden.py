@@ -5,6 +5,7 @@ reference's own headers (oracle/_ref/ref_piv); these tests need neither the
 reference nor a GPU.
 """
 import hashlib
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -132,3 +133,19 @@ def test_find_peaks_cases():
             for r, g in zip(ref, got):
                 if all_vals.count(r[2]) == 1:
                     assert (int(r[0]), int(r[1])) == (g[0], g[1])
+
+
+@pytest.mark.parametrize("case", ["u16", "u8", "pair", "same", "piv"])
+def test_average_subtract_matches_reference(case):
+    """examples/average_subtract/main.cpp:98-159 (mean of inputs 1..n-1,
+    subtract, PNM 16-bit scaling) against the reference's own pipeline:
+    bit-exact mean and saved images."""
+    g = np.load(Path(__file__).parent / "golden" / "avgsub_cases.npz")
+    frames = g[f"{case}_frames"]
+    mean, out = orc.average_subtract(frames)
+    assert np.array_equal(mean, g[f"{case}_mean"])
+    assert np.array_equal(out, g[f"{case}_out"])
+    if case == "pair":
+        assert np.array_equal(mean, frames[1].astype(np.float64))     # the first input is not averaged
+    if case == "same":
+        assert not out.any()                                         # d = 0, range 1
