@@ -163,6 +163,28 @@ typedef enum opiv_direction {
 int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t direction,
                                const double* in, double* out);
 
+/* ---- frame ingest: the image loaders ----
+ *
+ * loaders/image_loader.cpp:23-57 (format from the first bytes),
+ * pnm_image_loader.cpp:279-310 and tiff_image_loader.cpp:160-236 (read
+ * without libtiff): PNM P5 and baseline TIFF (uncompressed strips, one
+ * sample of 8 or 16 bits, every page), delivered in their own pixel type
+ * (OPIV_U8 / OPIV_U16) -- no widening.  Errors keep the reference's texts
+ * ("failed to open ...", "failed to find loader for ...", "image index out
+ * of range ..."). */
+typedef struct opiv_image_info {
+    uint32_t width, height;
+    int32_t dtype;           /* OPIV_U8 or OPIV_U16 */
+    uint32_t count;          /* images in the file (TIFF pages; 1 for PNM) */
+} opiv_image_info;
+
+int opiv_image_probe(const char* path, opiv_image_info* info);
+
+/* image `index` of path into dst (pitch elements per row, 0 = width) in the
+ * file's pixel type; dst host or device memory (device: copied on ctx's
+ * stream, ctx required; host: ctx may be NULL).  Returns when written. */
+int opiv_image_read(opiv_ctx* ctx, const char* path, uint32_t index, void* dst, size_t pitch);
+
 /* ---- background subtraction: examples/average_subtract/main.cpp:98-159 ----
  *
  * For n >= 2 frames of one size and pixel type (frames[i]: host or device,

@@ -9,10 +9,10 @@
 // here every window of the pair goes through one GPU launch, so they are
 // accepted and ignored.  --device picks the GPU.
 //
-// Frames are read with a PNM reader following the reference's loader
-// (loaders/pnm_image_loader.cpp:279-310; P5, 8- or 16-bit big-endian, '#'
-// comments before each header number) and are handed to the device as
-// 8/16-bit pixels, without the FP64 widening of the reference.
+// Frames are read by the engine's loaders (PNM P5 and baseline TIFF, the
+// formats of the reference's loader registry, loaders/image_loader.cpp:23-57)
+// and are handed to the device as 8/16-bit pixels, without the FP64
+// widening of the reference.
 //
 // Output matches main.cpp:312-313: one "x, y, u, v, sn" line per window in
 // grid order, at the stream's default precision.
@@ -47,52 +47,29 @@ struct Frame {
     const void* data() const { return wide ? (const void*)px16.data() : (const void*)px8.data(); }
 };
 
-std::istream& skip_comments(std::istream& is)
+// the engine's loaders (opiv_image_read: PNM P5 and baseline TIFF, the
+// formats of the reference's loader registry)
+bool load_frame(const std::string& path, Frame& f, std::string& err)
 {
-    // pnm_image_loader.cpp:39-48
-    while (is.peek() == '#') is.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
-    return is;
-}
-
-bool load_pnm(const std::string& path, Frame& f, std::string& err)
-{
-    std::ifstream is(path, std::ios::binary);
-    if (!is.is_open()) {
-        err = "failed to open " + path;
+    opiv_image_info info;
+    if (opiv_image_probe(path.c_str(), &info) != OPIV_OK) {
+        err = opiv_last_error();
         return false;
     }
-    char magic[1024] = {};
-    is.getline(magic, sizeof(magic));
-    if (!(magic[0] == 'P' && (magic[1] == '5' || magic[1] == '6'))) {
-        err = "failed to find loader for " + path;
-        return false;
-    }
-    if (magic[1] == '6') {
-        err = "failed to load image: " + path + " is colour (P6); only greyscale P5 frames are supported";
-        return false;
-    }
-    uint32_t depth = 0;
-    skip_comments(is) >> f.width;
-    skip_comments(is) >> f.height;
-    skip_comments(is) >> depth;
-    is.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
-    if (!is || f.width == 0 || f.height == 0) {
-        err = "failed to load image: bad PNM header in " + path;
-        return false;
-    }
+    f.width = info.width;
+    f.height = info.height;
+    f.wide = info.dtype == OPIV_U16;
     const size_t n = (size_t)f.width * f.height;
-    f.wide = depth > 255;
+    void* dst = nullptr;
     if (f.wide) {
-        std::vector<uint8_t> raw(2 * n);
-        is.read(reinterpret_cast<char*>(raw.data()), (std::streamsize)raw.size());
         f.px16.resize(n);
-        for (size_t i = 0; i < n; ++i) f.px16[i] = (uint16_t)((raw[2 * i] << 8) | raw[2 * i + 1]);   // big-endian
+        dst = f.px16.data();
     } else {
         f.px8.resize(n);
-        is.read(reinterpret_cast<char*>(f.px8.data()), (std::streamsize)n);
+        dst = f.px8.data();
     }
-    if (!is) {
-        err = "failed to load image: short pixel data in " + path;
+    if (opiv_image_read(nullptr, path.c_str(), 0, dst, 0) != OPIV_OK) {
+        err = opiv_last_error();
         return false;
     }
     return true;
@@ -190,7 +167,7 @@ int main(int argc, char* argv[])
 
     Frame fa, fb;
     std::string err;
-    if (!load_pnm(inputs[0], fa, err) || !load_pnm(inputs[1], fb, err)) {
+    if (!load_frame(inputs[0], fa, err) || !load_frame(inputs[1], fb, err)) {
         log(Level::ERROR, err);
         return 1;
     }

@@ -60,6 +60,7 @@ EXPORTS = (
     "opiv_batch_set_pairs", "opiv_batch_shape", "opiv_batch_run",
     "opiv_vjp_pair", "opiv_variance_pair",
     "opiv_average_subtract",
+    "opiv_image_probe", "opiv_image_read",
 )
 
 
@@ -85,6 +86,11 @@ class PlanDesc(ctypes.Structure):
 class PairDesc(ctypes.Structure):
     _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p),
                 ("img_w", ctypes.c_uint32), ("img_h", ctypes.c_uint32), ("pitch", ctypes.c_size_t)]
+
+
+class ImageInfo(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("dtype", ctypes.c_int32),
+                ("count", ctypes.c_uint32)]
 
 
 class BatchDesc(ctypes.Structure):
@@ -152,6 +158,8 @@ def load() -> ctypes.CDLL:
     lib.opiv_vjp_pair.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.opiv_variance_pair.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp]
     lib.opiv_average_subtract.argtypes = [vp, vp, sz, u32, u32, sz, i32, vp, vp]
+    lib.opiv_image_probe.argtypes = [ctypes.c_char_p, ctypes.POINTER(ImageInfo)]
+    lib.opiv_image_read.argtypes = [vp, ctypes.c_char_p, u32, vp, sz]
     for name in EXPORTS:
         if name not in ("opiv_last_error", "opiv_version", "opiv_ctx_stream"):
             getattr(lib, name).restype = ctypes.c_int
@@ -163,6 +171,21 @@ def _check(rc: int) -> None:
     if rc != 0:
         msg = load().opiv_last_error().decode()
         raise OpivError(rc, msg)
+
+
+def image_info(path) -> tuple[int, int, np.dtype, int]:
+    """(width, height, pixel dtype, images in the file) of a PNM/TIFF frame."""
+    info = ImageInfo()
+    _check(load().opiv_image_probe(str(path).encode(), ctypes.byref(info)))
+    return info.width, info.height, np.dtype(np.uint16 if info.dtype == U16 else np.uint8), info.count
+
+
+def read_image(path, index: int = 0) -> np.ndarray:
+    """Image `index` of a PNM/TIFF file as uint8/uint16 (h, w), as stored."""
+    w, h, dt, _ = image_info(path)
+    out = np.empty((h, w), dt)
+    _check(load().opiv_image_read(None, str(path).encode(), index, out.ctypes.data, 0))
+    return out
 
 
 def direct_chosen(win: int, dtype=np.float32) -> bool:
@@ -284,6 +307,10 @@ class Context:
         op = (ctypes.c_void_p * n)(*out_ptrs)
         _check(self._lib.opiv_average_subtract(self.handle, fp, n, w, h, pitch, DTYPE_CODES[np.dtype(dtype)],
                                                op, mean_ptr))
+
+    def read_image_device(self, path, dst_ptr: int, pitch: int = 0, index: int = 0) -> None:
+        """Image `index` of a PNM/TIFF file into device memory (its own pixel type)."""
+        _check(self._lib.opiv_image_read(self.handle, str(path).encode(), index, dst_ptr, pitch))
 
     def transform(self, z: np.ndarray, direction: int = FORWARD) -> np.ndarray:
         """FFT::transform (fft.h:86-119) of one or n stacked square windows

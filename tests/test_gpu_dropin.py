@@ -4,6 +4,7 @@ golden vectors and the oracle.  Same bar as test_gpu_parity: planes within
 1e-5 of their max, displacements within 1e-3 px (plus the 6-significant-digit
 print rounding of main.cpp:312-313)."""
 import subprocess
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -11,7 +12,7 @@ import pytest
 import openpiv_amd as oa
 import oracle as orc
 from conftest import load_npz
-from dropin_util import PROCESS_BIN, build_dropin_check, parse_process_output, write_pgm
+from dropin_util import AVGSUB_BIN, PROCESS_BIN, build_dropin_check, parse_process_output, write_pgm, write_tiff
 
 pytestmark = pytest.mark.gpu
 
@@ -132,3 +133,57 @@ def test_hipfft_transform_matches_oracle(dropin, tmp_path, direction):
     got = np.fromfile(tmp_path / "o.bin", dtype=np.complex128).reshape(32, 32)
     ref = orc.FFT(32).transform(z, forward=direction == "forward")
     assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("case", ["u16", "u8", "piv"])
+def test_average_subtract_front_end_files(tmp_path, case):
+    """opiv_average_subtract writes <input>.avg_sub.pnm files byte-identical to
+    those of the reference's examples/average_subtract (golden outputs of its
+    own loader, arithmetic and PNM writer); the second input is a TIFF."""
+    g = np.load(Path(__file__).parent / "golden" / "avgsub_cases.npz")
+    frames, outs = g[f"{case}_frames"], g[f"{case}_out"]
+    paths = []
+    for i, f in enumerate(frames):
+        p = tmp_path / (f"f{i}.tif" if i == 1 else f"f{i}.pgm")
+        (write_tiff(p, [f]) if i == 1 else write_pgm(p, f))
+        paths.append(p)
+    r = subprocess.run([str(AVGSUB_BIN)] + [str(p) for p in paths], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    h, w = frames.shape[1:]
+    for p, o in zip(paths, outs):
+        expect = b"P5\n# created by pnm_image_loader\n%d %d\n65535\n" % (w, h) + o.astype(">u2").tobytes()
+        assert Path(str(p) + ".avg_sub.pnm").read_bytes() == expect
+
+
+def test_average_subtract_front_end_needs_two_inputs(tmp_path):
+    write_pgm(tmp_path / "a.pgm", np.zeros((8, 8), np.uint8))
+    r = subprocess.run([str(AVGSUB_BIN), str(tmp_path / "a.pgm")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "require two or more input images" in r.stderr
+
+
+def test_process_reads_tiff_like_pgm(tmp_path):
+    """the process front-end on TIFF frames (big-endian, strips) gives the
+    same output as on the same frames as PGM"""
+    a, b = oa.synth_pair(320, 256, seed=33, i0=(1000.0, 60000.0), noise=50.0, params=(0.7, -1.9), dtype=np.uint16)
+    write_pgm(tmp_path / "a.pgm", a)
+    write_pgm(tmp_path / "b.pgm", b)
+    write_tiff(tmp_path / "a.tif", [a], big_endian=True, rows_per_strip=16)
+    write_tiff(tmp_path / "b.tif", [b], rows_per_strip=5)
+    outs = []
+    for ext in ("pgm", "tif"):
+        r = subprocess.run([str(PROCESS_BIN), str(tmp_path / f"a.{ext}"), str(tmp_path / f"b.{ext}")],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(r.stdout)
+    assert outs[0] == outs[1] and len(outs[0]) > 0
+
+
+def test_read_image_into_device_memory(tmp_path, gpu_ctx):
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 65536, (90, 70)).astype(np.uint16)
+    write_tiff(tmp_path / "a.tif", [a, a[::-1].copy()], rows_per_strip=9)
+    pitch = 80
+    buf = oa.DeviceBuffer(90 * pitch * 2)
+    gpu_ctx.read_image_device(tmp_path / "a.tif", buf.ptr, pitch=pitch, index=1)
+    got = buf.download(np.zeros((90, pitch), np.uint16))[:, :70]
+    assert np.array_equal(got, a[::-1])
