@@ -313,6 +313,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--win", type=int, default=32)
+    ap.add_argument("--step", type=int, default=0, help="grid step (default: half the window, 50%% overlap)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--path", choices=["auto", "fft", "direct"], default="auto",
                     help="correlation path: fused FP32 FFT, BF16-split MFMA direct (16^2), or the faster")
@@ -350,6 +351,8 @@ def main():
     win = args.win
     log(f"[bench] generating 4096x4096 {args.dtype} pair, win={win}")
     a, b, w, step = workload(win, args.dtype)
+    if args.step:
+        step = args.step
     ctx = oa.Context(0)
     np_dtype = {"f32": np.float32, "u16": np.uint16, "u8": np.uint8}[args.dtype]
     variants = {"auto": oa.AUTO, "fft": oa.CIRCULAR_FFT_FP32, "direct": oa.DIRECT_BF16_SPLIT}

@@ -177,14 +177,18 @@ def test_rotation_4096_f32(gpu_ctx, win, step):
     compare(res, ref, f"rot4096-w{win}")
 
 
-@pytest.mark.parametrize("win,step", [(16, 8), (32, 16)])
+@pytest.mark.parametrize("win,step", [(16, 8), (16, 4), (16, 12), (32, 16)])
 def test_uint16_4096(gpu_ctx, win, step):
-    say(f"uint16 4096 win={win}")
+    """4096^2 uint16, I0 ~ U(1000, 60000), noise 50, shift (0.4, 1.1): 16^2
+    windows at step 8 (261121 windows), at true 75% overlap (step 4, 1042441
+    windows) and at the reference's `-o 0.75` reading (step 12, 116281
+    windows), and 32^2 at step 16; every window against the FP64 oracle."""
+    say(f"uint16 4096 win={win} step={step}")
     a, b = oa.synth_pair(4096, 4096, seed=4, i0=(1000.0, 60000.0), noise=50.0, params=(0.4, 1.1),
                          dtype=np.uint16)
     plan = oa.Plan(gpu_ctx, 4096, 4096, win, step=(step, step), dtype=np.uint16)
     ref = orc.process_pair(a, b, win, step, step, threads=16)
-    compare(plan.run(a, b), ref, f"u16-4096-w{win}")
+    compare(plan.run(a, b), ref, f"u16-4096-w{win}-s{step}")
 
 
 def test_limit_search_and_real_against_oracle(gpu_ctx):
