@@ -178,13 +178,16 @@ __device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, i
     sfor<W - 3>([&](auto Hm) {
         constexpr int h = decltype(Hm)::value + 1;
         const float c = s[h];
-        const float nb = fmaxf(fmaxf(from_left(c), from_right(c)), fmaxf(s[h - 1], s[h + 1]));
-        const bool ismax = col_ok && h >= hlo && h <= hhi && c > nb;
-        const float mc = ismax ? c : -INFINITY;
+        // strict maximum over the 4 neighbours as compares only (no min/max
+        // of shuffled values, which would need canonicalising); the region
+        // test is data flow, so mc stays a select of canonical values
+        const bool ismax = (c > s[h - 1]) & (c > s[h + 1]) & (c > from_left(c)) & (c > from_right(c));
+        const bool inreg = col_ok & (h >= hlo) & (h <= hhi);
+        const float mc = (ismax & inreg) ? c : -INFINITY;
         const bool gt = mc > v1;                  // rows arrive in scan order: ties keep the first
         v2 = fmaxf(v2, fminf(mc, v1));
         h1 = gt ? h : h1;
-        v1 = gt ? mc : v1;
+        v1 = fmaxf(v1, mc);                       // = gt ? mc : v1 (a max keeps v1 provably canonical)
     });
     i1 = h1 * W + t;
     // top-2 merge of disjoint lane sets, ascending partner distance
@@ -214,11 +217,13 @@ __device__ __forceinline__ float group_max(float v)
 // Results of the ROUNDS windows of this lane group from their swapped planes
 // s[r][h] (column t of each): find_peaks, the FP64 fit with the 5 * ROUNDS
 // logs spread over the group's lanes, and the vector; or, with PLANES, the
-// planes themselves.
-template <int W, int ROUNDS, bool PLANES>
+// planes themselves.  fetch(r, row, col) reads the value of round r's plane
+// at (row, col) from LDS (any lane may read any position of its group's
+// planes); it supplies the stencil around each peak.
+template <int W, int ROUNDS, bool PLANES, class Fetch>
 __device__ __forceinline__ void finish_windows(const CorrParams& p, const float (&s)[ROUNDS][W],
                                                const WinGeom (&geo)[ROUNDS], const bool (&active)[ROUNDS],
-                                               int lane, int t)
+                                               int lane, int t, Fetch&& fetch)
 {
     const int gbase = lane - t;
     if constexpr (PLANES) {
@@ -240,7 +245,7 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
         const int hlo = r0 + 1, hhi = r0 + rlen - 3;
         const bool col_ok = t >= r0 + 1 && t <= r0 + rlen - 2;
 
-        float v1[ROUNDS], v2[ROUNDS], st[ROUNDS][5];
+        float v1[ROUNDS], v2[ROUNDS];
         int px[ROUNDS], py[ROUNDS];
         bool two[ROUNDS];
         sfor<ROUNDS>([&](auto R) {
@@ -250,22 +255,20 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
             two[r] = v2[r] > -INFINITY;
             px[r] = two[r] ? (i1 & (W - 1)) : 1;
             py[r] = two[r] ? (i1 / W) : 1;
-            // stencil (l, c, r, up, down) around the peak: the centre column
-            // comes from lane px, the horizontal neighbours from lanes px -+ 1
-            float mid = 0.f, up = 0.f, dn = 0.f;
-            sfor<W - 2>([&](auto Hm) {
-                constexpr int h = decltype(Hm)::value + 1;
-                const bool at = h == py[r];
-                mid = at ? s[r][h] : mid;
-                up = at ? s[r][h - 1] : up;
-                dn = at ? s[r][h + 1] : dn;
-            });
-            st[r][0] = __shfl(mid, gbase + px[r] - 1, 64);
-            st[r][1] = __shfl(mid, gbase + px[r], 64);
-            st[r][2] = __shfl(mid, gbase + px[r] + 1, 64);
-            st[r][3] = __shfl(up, gbase + px[r], 64);
-            st[r][4] = __shfl(dn, gbase + px[r], 64);
         });
+        // stencil value j of round r: (l, c, r, up, down) = plane positions
+        // (py, px - 1), (py, px), (py, px + 1), (py - 1, px), (py + 1, px)
+        auto stencil = [&](int r, int j) -> float {
+            int cx = px[0], cy = py[0];
+            sfor<ROUNDS - 1>([&](auto R) {
+                constexpr int q = decltype(R)::value + 1;
+                cx = r == q ? px[q] : cx;
+                cy = r == q ? py[q] : cy;
+            });
+            cx += j == 0 ? -1 : (j == 2 ? 1 : 0);
+            cy += j == 3 ? -1 : (j == 4 ? 1 : 0);
+            return fetch(r, cy, cx);
+        };
 
         // ---- FP64 logs of the 5 * ROUNDS stencil values, one per lane:
         // value j of round r sits on lane j * ROUNDS + r of the group ----
@@ -274,12 +277,9 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
         double lg[NP];
         sfor<NP>([&](auto K) {
             constexpr int k = decltype(K)::value;
-            float v = 1.f;
-            sfor<NV>([&](auto J) {
-                constexpr int L = decltype(J)::value;
-                v = (t + k * W == L) ? st[L % ROUNDS][L / ROUNDS] : v;
-            });
-            lg[k] = log((double)v);
+            const int L = t + k * W;
+            const float v = stencil(L % ROUNDS, L < NV ? L / ROUNDS : 1);
+            lg[k] = log((double)(L < NV ? v : 1.f));
         });
 
         // ---- lane t < ROUNDS of the group fits window t ----
@@ -313,21 +313,21 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
             });
         }
         bool act[ROUNDS], tw[ROUNDS];
-        float nmx[ROUNDS];
         int gx0[ROUNDS], gy0[ROUNDS], gout[ROUNDS];
         uint32_t gh[ROUNDS];
         sfor<ROUNDS>([&](auto R) {
             constexpr int r = decltype(R)::value;
             act[r] = active[r];
             tw[r] = two[r];
-            nmx[r] = fmaxf(fmaxf(st[r][0], st[r][2]), fmaxf(st[r][3], st[r][4]));
             gx0[r] = geo[r].x0;
             gy0[r] = geo[r].y0;
             gout[r] = geo[r].out;
             gh[r] = geo[r].img_h;
         });
         if (t >= ROUNDS || !pick(act)) return;
-        const float fv1 = pick(v1), fv2 = pick(v2), nmax = pick(nmx);
+        // the four stencil neighbours of the peak (near-tie test)
+        const float nmax = fmaxf(fmaxf(stencil(rr, 0), stencil(rr, 2)), fmaxf(stencil(rr, 3), stencil(rr, 4)));
+        const float fv1 = pick(v1), fv2 = pick(v2);
         const int fpx = pick(px), fpy = pick(py);
         const int x0 = pick(gx0), y0 = pick(gy0);
         opiv_window_result res;

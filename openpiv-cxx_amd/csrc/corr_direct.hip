@@ -284,7 +284,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
             s[r][h] = lds_p[r * WPW + slot][h][t];
         });
     });
-    finish_windows<W, ROUNDS, PLANES>(p, s, geo, active, lane, t);
+    finish_windows<W, ROUNDS, PLANES>(p, s, geo, active, lane, t, [&](int r, int row, int col) {
+        return lds_p[r * WPW + slot][row][col];
+    });
 }
 
 template <int DT>

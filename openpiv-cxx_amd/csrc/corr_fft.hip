@@ -50,22 +50,24 @@ using fftreg::sfor;
 // two packed spectra (zmax0 is the first window's), so each plane's FP32
 // rounding stays relative to its own magnitude; scaling by 2^k is exact and
 // the caller divides it out of the second plane.
+// column t of both frames of window g: z[y] = a + i b
+template <int W, int DT>
+__device__ __forceinline__ void load_column(const WinGeom& g, int t, float2 (&z)[W])
+{
+    const size_t idx = (size_t)g.y0 * g.pitch + (size_t)(g.x0 + t);
+    sfor<W>([&](auto Y) {
+        constexpr int y = decltype(Y)::value;
+        z[y] = make_float2(load_px<DT>(g.a, idx + (size_t)y * g.pitch), load_px<DT>(g.b, idx + (size_t)y * g.pitch));
+    });
+}
+
+// z: the window's column t (load_column), consumed
 template <int W, int DT, bool ACC>
-__device__ __forceinline__ float forward_product(const WinGeom& g, bool real_compat, int t,
+__device__ __forceinline__ float forward_product(float2 (&z)[W], bool real_compat, int t,
                                                  float2 (*lds)[W + 1], float2 (&q)[W],
                                                  float zmax0, float& scale)
 {
     constexpr int H2 = W / 2;
-    // ---- load column t: z[y] = a + i b ----
-    float2 z[W];
-    {
-        const size_t idx = (size_t)g.y0 * g.pitch + (size_t)(g.x0 + t);
-        sfor<W>([&](auto Y) {
-            constexpr int y = decltype(Y)::value;
-            z[y] = make_float2(load_px<DT>(g.a, idx + (size_t)y * g.pitch),
-                               load_px<DT>(g.b, idx + (size_t)y * g.pitch));
-        });
-    }
     // ---- FFT along y (column t), transpose through LDS ----
     fftreg::fft_dif<W, -1>(z);
     wave_sync();                                  // earlier readers of this buffer are done
@@ -143,13 +145,21 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, float2 
     });
 
     // ---- forward transforms; with two rounds the spectra pack as
-    // Q = P0 + i P1, whose inverse is c0 + i c1 (both planes are real) ----
+    // Q = P0 + i P1, whose inverse is c0 + i c1 (both planes are real).
+    // At W = 32 (two waves per SIMD) the second window's columns are loaded
+    // before the first window is transformed, so their latency hides behind
+    // that work. ----
+    constexpr bool PREFETCH = ROUNDS == 2 && W == 32;
     float2 q[W];
     float scale = 1.f, inv_scale = 1.f;
     const bool compat = p.real_compat != 0;
-    const float zmax0 = forward_product<W, DT, false>(geo[0], compat, t, lds, q, 0.f, scale);
+    float2 z0[W], z1[W];
+    load_column<W, DT>(geo[0], t, z0);
+    if constexpr (PREFETCH) load_column<W, DT>(geo[ROUNDS - 1], t, z1);
+    const float zmax0 = forward_product<W, DT, false>(z0, compat, t, lds, q, 0.f, scale);
     if constexpr (ROUNDS == 2) {
-        forward_product<W, DT, true>(geo[ROUNDS - 1], compat, t, lds, q, zmax0, scale);
+        if constexpr (!PREFETCH) load_column<W, DT>(geo[ROUNDS - 1], t, z1);
+        forward_product<W, DT, true>(z1, compat, t, lds, q, zmax0, scale);
         inv_scale = 1.f / scale;
     }
 
@@ -177,7 +187,23 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, float2 
         if constexpr (ROUNDS == 2) s[ROUNDS - 1][h] = v.y * inv_scale;
     });
 
-    finish_windows<W, ROUNDS, PLANES>(p, s, geo, active, lane, t);
+    // the planes to LDS (round r, column t, row h; the wave's transposes are
+    // done) so that the stencil around each peak is read by address
+    float* pl = reinterpret_cast<float*>(&lds[0][0]);
+    if constexpr (!PLANES) {
+        wave_sync();
+        sfor<ROUNDS>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            sfor<W>([&](auto Hh) {
+                constexpr int h = decltype(Hh)::value;
+                pl[(r * W + t) * (W + 1) + h] = s[r][h];
+            });
+        });
+        wave_sync();
+    }
+    finish_windows<W, ROUNDS, PLANES>(p, s, geo, active, lane, t, [&](int r, int row, int col) {
+        return pl[(r * W + col) * (W + 1) + row];
+    });
 }
 
 // occupancy target per window size: the VGPR budget that keeps the
