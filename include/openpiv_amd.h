@@ -57,7 +57,8 @@ typedef enum opiv_variant {
     OPIV_DIRECT_BF16_SPLIT = 2,
     /* the faster of OPIV_CIRCULAR_FFT_FP32 and OPIV_DIRECT_BF16_SPLIT for
      * the plan's window size and pixel type as measured on MI355X (same
-     * semantics); at present OPIV_CIRCULAR_FFT_FP32 for every size and type */
+     * semantics): the direct path for 8-bit frames at 16x16, the FFT path
+     * otherwise */
     OPIV_AUTO = 3
 } opiv_variant;
 

@@ -121,15 +121,13 @@ int opiv_grid(uint32_t img_w, uint32_t img_h, uint32_t win_w, uint32_t win_h,
 // same circular correlation): the MFMA direct path for 16 x 16 windows, the
 // fused FFT otherwise
 // OPIV_AUTO picks the faster path measured on MI355X (kernel traces, 4096^2
-// pairs, 16 x 16 windows at step 8): the FFT kernel runs 112-116 us for u8,
-// u16 and f32 frames; the direct MFMA kernel ties it for u8 (111-117 us) and
-// is slower for u16 (208-218 us) and f32 (311-345 us).  So AUTO is the FFT
-// kernel; the direct path stays available as OPIV_DIRECT_BF16_SPLIT.
+// pairs, 16 x 16 windows at step 8): for 8-bit frames the direct MFMA kernel
+// (105.6 us, and its correlation is exact) against the FFT kernel's 109.4 us;
+// for 16-bit and float frames the FFT kernel (110 / 109 us against 200 / 312
+// us direct).  Other window sizes have no direct kernel.
 static int32_t auto_variant(int win, int dtype)
 {
-    (void)win;
-    (void)dtype;
-    return OPIV_CIRCULAR_FFT_FP32;
+    return opiv::direct_supported(win) && dtype == OPIV_U8 ? OPIV_DIRECT_BF16_SPLIT : OPIV_CIRCULAR_FFT_FP32;
 }
 
 int opiv_plan_create(opiv_ctx* ctx, const opiv_plan_desc* desc, opiv_plan** out)

@@ -212,6 +212,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
         constexpr int NACC = NP == 1 ? 2 : (NP == 2 ? 4 : 6);
         floatx4 part[NACC];
         sfor<NACC>([&](auto Q) { part[decltype(Q)::value] = floatx4{0.f, 0.f, 0.f, 0.f}; });
+        // For u8 (one term, LDS-read bound) the A operand of chunk c is that of
+        // chunk c - 1 moved two lanes along each 16-lane row (row i16 of P at
+        // rows y = 2c + g/2 reads a[(y - i16) mod 16]): DPP row_ror:2 instead
+        // of an LDS read.
+        constexpr bool AROT = NP == 1;
+        uint32_t Aw[NP][4];
+        if constexpr (AROT) {
+            const uint4 v = *reinterpret_cast<const uint4*>(&lds_a[0][((g >> 1) - i16) & (W - 1)][8 * (g & 1)]);
+            Aw[0][0] = v.x;
+            Aw[0][1] = v.y;
+            Aw[0][2] = v.z;
+            Aw[0][3] = v.w;
+        }
         sfor<8>([&](auto Cc) {
             constexpr int c = decltype(Cc)::value;
             const int ra = (2 * c + (g >> 1) - i16) & (W - 1);
@@ -221,7 +234,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
             bf16x8 A[NP], B[NP];
             sfor<NP>([&](auto K) {
                 constexpr int k = decltype(K)::value;
-                A[k] = *reinterpret_cast<const bf16x8*>(&lds_a[k][ra][8 * (g & 1)]);
+                if constexpr (AROT) {
+                    if constexpr (c > 0) {
+                        sfor<4>([&](auto Mm) {
+                            constexpr int m = decltype(Mm)::value;
+                            Aw[k][m] = (uint32_t)__builtin_amdgcn_mov_dpp((int)Aw[k][m], 0x122, 0xf, 0xf, false);
+                        });
+                    }
+                    A[k] = __builtin_bit_cast(bf16x8, Aw[k]);
+                } else {
+                    A[k] = *reinterpret_cast<const bf16x8*>(&lds_a[k][ra][8 * (g & 1)]);
+                }
                 const uint32_t* q = reinterpret_cast<const uint32_t*>(&lds_b[k][copy][rb * 2 * W + base]);
                 uint32_t w4[4] = {q[0], q[1], q[2], q[3]};
                 B[k] = __builtin_bit_cast(bf16x8, w4);
