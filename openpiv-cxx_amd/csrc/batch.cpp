@@ -171,7 +171,8 @@ int upload_table(opiv_batch* b)
     const size_t bytes = b->table.size() * sizeof(opiv::BatchPair);
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));   // no graph in flight reads the old table
     HIP_TRY(b->d_table.ensure(bytes));
-    HIP_TRY(hipMemcpy(b->d_table.p, b->table.data(), bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(b->d_table.p, b->table.data(), bytes, hipMemcpyHostToDevice, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));   // the host table may change once this returns
     return OPIV_OK;
 }
 

@@ -495,7 +495,11 @@ def _hip_runtime():
 
 
 class DeviceBuffer:
-    """Device allocation (hipMalloc) with host copies; ``ptr`` is the device address."""
+    """Device allocation (hipMalloc) with host copies; ``ptr`` is the device address.
+
+    ``upload`` and ``zero`` run on the null stream and return only once the
+    device has finished them: a context's stream is non-blocking, so work it
+    is given afterwards is not otherwise ordered after them."""
 
     H2D, D2H = 1, 2
 
@@ -520,6 +524,8 @@ class DeviceBuffer:
         assert a.nbytes <= self.nbytes
         if self._h.hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, self.H2D) != 0:
             raise OpivError(5, "hipMemcpy H2D failed")
+        if self._h.hipDeviceSynchronize() != 0:
+            raise OpivError(5, "hipDeviceSynchronize failed")
 
     def download(self, out: np.ndarray) -> np.ndarray:
         assert out.flags.c_contiguous and out.nbytes <= self.nbytes
@@ -530,6 +536,8 @@ class DeviceBuffer:
     def zero(self) -> None:
         if self._h.hipMemset(self.ptr, 0, self.nbytes) != 0:
             raise OpivError(5, "hipMemset failed")
+        if self._h.hipDeviceSynchronize() != 0:
+            raise OpivError(5, "hipDeviceSynchronize failed")
 
     def free(self) -> None:
         if self.ptr:
