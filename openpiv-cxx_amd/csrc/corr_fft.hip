@@ -29,6 +29,8 @@
 // LDS traffic stays inside a wave, so only wave-level syncs are needed.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cmath>
 #include <cstdint>
 #include <limits>
@@ -61,33 +63,72 @@ __device__ __forceinline__ void load_column(const WinGeom& g, int t, float2 (&z)
     });
 }
 
+// At 64 x 64 the lane group's LDS slice holds floats, and every complex
+// exchange goes through it in two passes (real parts, then imaginary parts):
+// half the LDS, so two waves fit per SIMD.
+template <int W>
+constexpr bool split_lds() { return W == 64; }
+template <int W>
+using LdsT = std::conditional_t<split_lds<W>(), float, float2>;
+
+// Complex exchange through the lane group's LDS slice: this lane's values
+// src(i) go to lds[wi(i)], then dst(j, value) receives lds[ri(j)] (i, j =
+// 0..W-1, compile-time).  The caller has synchronised earlier readers of the
+// slice; src is fully read before dst is called.
+template <int W, class Src, class Wi, class Ri, class Dst>
+__device__ __forceinline__ void lds_exchange(LdsT<W> (*lds)[W + 1], Src&& src, Wi&& wi, Ri&& ri, Dst&& dst)
+{
+    if constexpr (!split_lds<W>()) {
+        sfor<W>([&](auto I) {
+            const int2 a = wi(I);
+            lds[a.x][a.y] = src(I);
+        });
+        wave_sync();
+        sfor<W>([&](auto J) {
+            const int2 a = ri(J);
+            dst(J, lds[a.x][a.y]);
+        });
+    } else {
+        float re[W];
+        sfor<W>([&](auto I) {
+            const int2 a = wi(I);
+            lds[a.x][a.y] = src(I).x;
+        });
+        wave_sync();
+        sfor<W>([&](auto J) {
+            const int2 a = ri(J);
+            re[decltype(J)::value] = lds[a.x][a.y];
+        });
+        wave_sync();
+        sfor<W>([&](auto I) {
+            const int2 a = wi(I);
+            lds[a.x][a.y] = src(I).y;
+        });
+        wave_sync();
+        sfor<W>([&](auto J) {
+            const int2 a = ri(J);
+            dst(J, make_float2(re[decltype(J)::value], lds[a.x][a.y]));
+        });
+    }
+}
+
 // z: the window's column t (load_column), consumed
 template <int W, int DT, bool ACC>
 __device__ __forceinline__ float forward_product(float2 (&z)[W], bool real_compat, int t,
-                                                 float2 (*lds)[W + 1], float2 (&q)[W],
+                                                 LdsT<W> (*lds)[W + 1], float2 (&q)[W],
                                                  float zmax0, float& scale)
 {
     constexpr int H2 = W / 2;
     // ---- FFT along y (column t), transpose through LDS ----
     fftreg::fft_dif<W, -1>(z);
     wave_sync();                                  // earlier readers of this buffer are done
-    sfor<W>([&](auto K) {
-        constexpr int ky = decltype(K)::value;
-        lds[ky][t] = z[brev_n<W>(ky)];
-    });
-    wave_sync();
-    sfor<W>([&](auto X) {
-        constexpr int x = decltype(X)::value;
-        z[x] = lds[t][x];
-    });
-    // ---- FFT along x: lane t holds spectral row ky = t; publish it ----
+    lds_exchange<W>(
+        lds, [&](auto K) { return z[brev_n<W>(decltype(K)::value)]; },
+        [&](auto K) { return make_int2(decltype(K)::value, t); },
+        [&](auto X) { return make_int2(t, decltype(X)::value); },
+        [&](auto X, float2 v) { z[decltype(X)::value] = v; });
+    // ---- FFT along x: lane t holds spectral row ky = t ----
     fftreg::fft_dif<W, -1>(z);
-    wave_sync();
-    sfor<W>([&](auto K) {
-        constexpr int kx = decltype(K)::value;
-        lds[t][kx] = z[brev_n<W>(kx)];
-    });
-    wave_sync();
     // max |Z| over the window (lanes of the group)
     float zmax = 0.f;
     sfor<W>([&](auto K) {
@@ -104,14 +145,13 @@ __device__ __forceinline__ float forward_product(float2 (&z)[W], bool real_compa
         scale = ldexpf(1.f, k);
     }
     const float f = 0.25f * scale;
-    // ---- separate the two real spectra and form P = B conj(A) ----
+    // ---- separate the two real spectra and form P = B conj(A) from the
+    // row's spectrum and the mirror row published in LDS ----
     const int m = (W - t) & (W - 1);              // row of -ky
     const bool compat_row = real_compat && (t == 0 || t == H2);
-    sfor<W>([&](auto K) {
+    auto product = [&](auto K, float2 zr) {       // zr = Z(-k)
         constexpr int kx = decltype(K)::value;
-        constexpr int kxm = (W - kx) & (W - 1);
         const float2 zk = z[brev_n<W>(kx)];
-        const float2 zr = lds[m][kxm];            // Z(-k)
         // s = Z + conj Z(-k) = 2A ; d = Z - conj Z(-k) = 2iB ; P = -i d conj(s) / 4
         const float sr = zk.x + zr.x, si = zk.y - zr.y;
         const float dr = zk.x - zr.x, di = zk.y + zr.y;
@@ -121,14 +161,19 @@ __device__ __forceinline__ float forward_product(float2 (&z)[W], bool real_compa
         if (compat_row || (kx == 0 && real_compat)) { pr = 0.f; pi = 0.f; }
         if constexpr (ACC) q[kx] = make_float2(q[kx].x - pi, q[kx].y + pr);
         else q[kx] = make_float2(pr, pi);
-    });
+    };
+    wave_sync();
+    lds_exchange<W>(
+        lds, [&](auto K) { return z[brev_n<W>(decltype(K)::value)]; },
+        [&](auto K) { return make_int2(t, decltype(K)::value); },
+        [&](auto K) { return make_int2(m, (W - decltype(K)::value) & (W - 1)); }, product);
     return zmax;
 }
 
 // One wave task: ROUNDS x (64 / W) windows, windows
 // [(task * ROUNDS + r) * 64/W + slot] for round r and lane group slot.
 template <int W, int DT, bool PLANES, int ROUNDS, bool BATCH>
-__device__ __forceinline__ void corr_task(const CorrParams& p, int task, float2 (*lds)[W + 1], int lane)
+__device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W> (*lds)[W + 1], int lane)
 {
     constexpr int WPW = 64 / W;                 // lane groups per wave
     constexpr int H2 = W / 2;
@@ -166,16 +211,12 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, float2 
     // ---- inverse along x (row ky = t), transpose, inverse along y ----
     fftreg::fft_dif<W, +1>(q);
     wave_sync();
-    sfor<W>([&](auto X) {
-        constexpr int x = decltype(X)::value;
-        lds[t][x] = q[brev_n<W>(x)];
-    });
-    wave_sync();
     const int xs = (t + H2) & (W - 1);           // plane column that swaps into column t
-    sfor<W>([&](auto K) {
-        constexpr int ky = decltype(K)::value;
-        q[ky] = lds[ky][xs];
-    });
+    lds_exchange<W>(
+        lds, [&](auto X) { return q[brev_n<W>(decltype(X)::value)]; },
+        [&](auto X) { return make_int2(t, decltype(X)::value); },
+        [&](auto K) { return make_int2(decltype(K)::value, xs); },
+        [&](auto K, float2 v) { q[decltype(K)::value] = v; });
     fftreg::fft_dif<W, +1>(q);
 
     // quadrant-swapped planes, column t: S[h] = c[(h + W/2) mod W]
@@ -210,7 +251,7 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, float2 
 // register-resident lines (nearly) without spilling; batch kernels carry
 // per-lane frame pointers and get a little more room at W = 16
 template <int W, bool BATCH>
-constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (BATCH ? 3 : 4) : (W == 32 ? 2 : 1); }
+constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (BATCH ? 3 : 4) : 2; }
 
 // BATCH reads each window's pair from the device table (per-lane frame
 // pointers) and loops over wave tasks, so the launch may be persistent; a
@@ -221,7 +262,7 @@ void corr_fft_kernel(CorrParams p)
 {
     constexpr int WPW = 64 / W;
     constexpr int WAVES = NT / 64;
-    __shared__ float2 lds_all[WAVES][WPW][W][W + 1];
+    __shared__ LdsT<W> lds_all[WAVES][WPW][W][W + 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int ntasks = (p.n_windows + ROUNDS * WPW - 1) / (ROUNDS * WPW);
