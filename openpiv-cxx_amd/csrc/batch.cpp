@@ -139,6 +139,9 @@ int fill_table(opiv_batch* b, const opiv_pair_desc* pairs, size_t npairs)
             if (!pr.a || !pr.b) return fail(OPIV_E_INVALID, "pair %zu: null frame pointer", i);
             const size_t pitch = pr.pitch ? pr.pitch : pr.img_w;
             if (pitch < pr.img_w) return fail(OPIV_E_INVALID, "pair %zu: pitch smaller than width", i);
+            if (pitch * dtype_size(d.dtype) > opiv::kMaxPitchBytes)
+                return fail(OPIV_E_UNSUPPORTED, "pair %zu: row pitch of %zu bytes exceeds %zu", i,
+                            pitch * dtype_size(d.dtype), opiv::kMaxPitchBytes);
             uint32_t nx, ny;
             rc = opiv_grid(pr.img_w, pr.img_h, win, win, ox, oy, &nx, &ny, nullptr);
             if (rc) return fail(rc, "pair %zu, pass %u: %s", i, ps, opiv_last_error());

@@ -146,6 +146,9 @@ int opiv_plan_create(opiv_ctx* ctx, const opiv_plan_desc* desc, opiv_plan** out)
         return fail(OPIV_E_UNSUPPORTED, "the direct BF16 path supports 16x16 windows, not %u", win);
     const size_t pitch = desc->pitch ? desc->pitch : desc->img_w;
     if (pitch < desc->img_w) return fail(OPIV_E_INVALID, "pitch %zu smaller than width %u", pitch, desc->img_w);
+    if (pitch * dtype_size(desc->dtype) > opiv::kMaxPitchBytes)
+        return fail(OPIV_E_UNSUPPORTED, "row pitch of %zu bytes exceeds %zu", pitch * dtype_size(desc->dtype),
+                    opiv::kMaxPitchBytes);
 
     opiv_plan* p = new opiv_plan;
     p->ctx = ctx;

@@ -40,6 +40,24 @@ __device__ __forceinline__ float from_left(float v)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 
+// max(value of lane - 1, value of lane + 1, up, down): the left neighbour
+// enters through DPP as an operand of the max itself.  Written out because a
+// max of a DPP result in compiled code first canonicalises it (one more
+// instruction).  s_nop 1 gives the two wait states a DPP read needs after a
+// VALU write of its source.  Group-edge lanes take a neighbour group's value
+// or zero, as with from_left / from_right.
+__device__ __forceinline__ float neighbour_max(float c, float up, float down)
+{
+    const float r = from_right(c);
+    float m;
+    asm("s_nop 1\n\t"
+        "v_max_f32_dpp %0, %1, %2 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:0\n\t"
+        "v_max3_f32 %0, %0, %3, %4"
+        : "=&v"(m)
+        : "v"(c), "v"(r), "v"(up), "v"(down));
+    return m;
+}
+
 // Partner exchange for reductions over lane groups of W lanes, applied with
 // OFF = 1, 2, 4, ... W/2 in that order: lane ^ 1 and lane ^ 2 (quad_perm),
 // the mirror lane of the 8-lane half (row_half_mirror), the mirror lane of the
@@ -178,14 +196,13 @@ __device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, i
     sfor<W - 3>([&](auto Hm) {
         constexpr int h = decltype(Hm)::value + 1;
         const float c = s[h];
-        // strict maximum over the 4 neighbours as compares only (no min/max
-        // of shuffled values, which would need canonicalising); the region
-        // test is data flow, so mc stays a select of canonical values
-        const bool ismax = (c > s[h - 1]) & (c > s[h + 1]) & (c > from_left(c)) & (c > from_right(c));
+        // strict maximum over the 4 neighbours; the region test is data
+        // flow, so mc stays a select of canonical values
+        const bool ismax = c > neighbour_max(c, s[h - 1], s[h + 1]);
         const bool inreg = col_ok & (h >= hlo) & (h <= hhi);
         const float mc = (ismax & inreg) ? c : -INFINITY;
         const bool gt = mc > v1;                  // rows arrive in scan order: ties keep the first
-        v2 = fmaxf(v2, fminf(mc, v1));
+        v2 = __builtin_amdgcn_fmed3f(v2, mc, v1); // = max(v2, min(mc, v1)), as v2 <= v1
         h1 = gt ? h : h1;
         v1 = fmaxf(v1, mc);                       // = gt ? mc : v1 (a max keeps v1 provably canonical)
     });
@@ -217,11 +234,15 @@ __device__ __forceinline__ float group_max(float v)
 // Results of the ROUNDS windows of this lane group from their swapped planes
 // s[r][h] (column t of each): find_peaks, the FP64 fit with the 5 * ROUNDS
 // logs spread over the group's lanes, and the vector; or, with PLANES, the
-// planes themselves.  fetch(r, row, col) reads the value of round r's plane
+// planes themselves.  Plane r is the correlation divided by unscale[r], a
+// power of two: the peak order, the tie tests and the log-ratio fit do not
+// depend on it, and it is multiplied back (exactly) into the reported peak
+// values and planes.  fetch(r, row, col) reads the value of round r's plane
 // at (row, col) from LDS (any lane may read any position of its group's
 // planes); it supplies the stencil around each peak.
 template <int W, int ROUNDS, bool PLANES, class Fetch>
 __device__ __forceinline__ void finish_windows(const CorrParams& p, const float (&s)[ROUNDS][W],
+                                               const float (&unscale)[ROUNDS],
                                                const WinGeom (&geo)[ROUNDS], const bool (&active)[ROUNDS],
                                                int lane, int t, Fetch&& fetch)
 {
@@ -233,7 +254,7 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
                 double* dst = p.planes + (size_t)geo[r].out * W * W + t;
                 sfor<W>([&](auto Hh) {
                     constexpr int h = decltype(Hh)::value;
-                    dst[h * W] = (double)s[r][h];
+                    dst[h * W] = (double)(s[r][h] * unscale[r]);
                 });
             }
         });
@@ -326,8 +347,9 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
         });
         if (t >= ROUNDS || !pick(act)) return;
         // the four stencil neighbours of the peak (near-tie test)
-        const float nmax = fmaxf(fmaxf(stencil(rr, 0), stencil(rr, 2)), fmaxf(stencil(rr, 3), stencil(rr, 4)));
-        const float fv1 = pick(v1), fv2 = pick(v2);
+        const float us = pick(unscale);
+        const float nmax = us * fmaxf(fmaxf(stencil(rr, 0), stencil(rr, 2)), fmaxf(stencil(rr, 3), stencil(rr, 4)));
+        const float fv1 = us * pick(v1), fv2 = us * pick(v2);
         const int fpx = pick(px), fpy = pick(py);
         const int x0 = pick(gx0), y0 = pick(gy0);
         opiv_window_result res;

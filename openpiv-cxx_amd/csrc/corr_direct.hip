@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
         constexpr int i = decltype(I)::value;
         sfor<4>([&](auto R) {
             constexpr int r = decltype(R)::value;
-            lds_p[i][(4 * g + r + 8) & (W - 1)][wcol] = accs[i][r] * (float)(W * W);
+            lds_p[i][(4 * g + r + 8) & (W - 1)][wcol] = accs[i][r];
         });
     });
     wave_sync();
@@ -307,7 +307,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
             s[r][h] = lds_p[r * WPW + slot][h][t];
         });
     });
-    finish_windows<W, ROUNDS, PLANES>(p, s, geo, active, lane, t, [&](int r, int row, int col) {
+    // the sums are the correlation divided by W * W (the reference's
+    // unnormalised inverse FFT), an exact power of two that the tail restores
+    float unscale[ROUNDS];
+    sfor<ROUNDS>([&](auto R) { unscale[decltype(R)::value] = (float)(W * W); });
+    finish_windows<W, ROUNDS, PLANES>(p, s, unscale, geo, active, lane, t, [&](int r, int row, int col) {
         return lds_p[r * WPW + slot][row][col];
     });
 }

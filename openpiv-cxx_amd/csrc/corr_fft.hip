@@ -44,22 +44,63 @@ namespace opiv {
 using fftreg::brev_n;
 using fftreg::sfor;
 
-// Load window (x0, y0) column t of both frames, transform, and leave in q
-// (natural order, lane t = spectral row ky) the cross-power spectrum
-// P = B conj(A) of this lane's row -- or, with ACC, add i 2^k P to q.
-// Returns the window's max |Z| component (Z = FFT(a + i b)), which bounds
-// |P| <= max|Z|^2.  The power of two 2^k equalises that bound between the
-// two packed spectra (zmax0 is the first window's), so each plane's FP32
-// rounding stays relative to its own magnitude; scaling by 2^k is exact and
-// the caller divides it out of the second plane.
+// The frames of one wave task as buffers.  All windows of a task lie in one
+// frame pair, and the element offset y0 * pitch + x0 grows with the window
+// index, so lane 0's first window has the task's smallest: the buffers start
+// at the frames' pixel there (wave-uniform), a lane adds its own window and
+// column as a 32-bit byte offset, and row y adds y * pitch bytes as a
+// uniform offset -- no per-lane 64-bit address arithmetic per load.
+struct TaskFrames {
+    __amdgpu_buffer_rsrc_t a, b;
+    uint64_t base;                // element offset of the buffer starts
+    uint32_t pitch_bytes;
+};
+
+template <int DT>
+constexpr uint32_t px_bytes() { return DT == 0 ? 1 : DT == 1 ? 2 : DT == 2 ? 4 : 8; }
+
+__device__ __forceinline__ uint64_t first_lane(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// g0: this lane's first window; called with every lane of the wave active
+template <int DT>
+__device__ __forceinline__ TaskFrames task_frames(const WinGeom& g0)
+{
+    constexpr uint32_t ES = px_bytes<DT>();
+    constexpr int kRawWord3 = 0x00020000;       // 32-bit data format, the untyped-access setting
+    TaskFrames f;
+    f.base = first_lane((uint64_t)g0.y0 * g0.pitch + (uint64_t)g0.x0);
+    const uint64_t pitch = first_lane(g0.pitch);
+    const uint64_t a = first_lane((uint64_t)(uintptr_t)g0.a) + f.base * ES;
+    const uint64_t b = first_lane((uint64_t)(uintptr_t)g0.b) + f.base * ES;
+    f.a = __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)a, 0, -1, kRawWord3);
+    f.b = __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)b, 0, -1, kRawWord3);
+    f.pitch_bytes = (uint32_t)(pitch * ES);
+    return f;
+}
+
+template <int DT>
+__device__ __forceinline__ float load_buf(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+{
+    if constexpr (DT == 0) return (float)__builtin_amdgcn_raw_buffer_load_b8(r, off, soff, 0);
+    else if constexpr (DT == 1) return (float)__builtin_amdgcn_raw_buffer_load_b16(r, off, soff, 0);
+    else if constexpr (DT == 2) return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+    else return (float)__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+
 // column t of both frames of window g: z[y] = a + i b
 template <int W, int DT>
-__device__ __forceinline__ void load_column(const WinGeom& g, int t, float2 (&z)[W])
+__device__ __forceinline__ void load_column(const TaskFrames& f, const WinGeom& g, int t, float2 (&z)[W])
 {
-    const size_t idx = (size_t)g.y0 * g.pitch + (size_t)(g.x0 + t);
+    const uint32_t off = (uint32_t)(((uint64_t)g.y0 * g.pitch + (uint64_t)(g.x0 + t) - f.base) * px_bytes<DT>());
     sfor<W>([&](auto Y) {
-        constexpr int y = decltype(Y)::value;
-        z[y] = make_float2(load_px<DT>(g.a, idx + (size_t)y * g.pitch), load_px<DT>(g.b, idx + (size_t)y * g.pitch));
+        constexpr uint32_t y = decltype(Y)::value;
+        const uint32_t so = y * f.pitch_bytes;
+        z[y] = make_float2(load_buf<DT>(f.a, off, so), load_buf<DT>(f.b, off, so));
     });
 }
 
@@ -112,13 +153,21 @@ __device__ __forceinline__ void lds_exchange(LdsT<W> (*lds)[W + 1], Src&& src, W
     }
 }
 
-// z: the window's column t (load_column), consumed
+// Transform the window's column t (z, from load_column; consumed) and leave
+// in q (natural order, lane t = spectral row ky) 4 P, P = B conj(A) the
+// cross-power spectrum of this lane's row -- or, with ACC, add i 2^k 4 P to
+// q.  Returns the window's max |Z| component (Z = FFT(a + i b)), which
+// bounds |P| <= max|Z|^2.  The power of two 2^k equalises that bound between
+// the two packed spectra (zmax0 is the first window's), so each plane's FP32
+// rounding stays relative to its own magnitude.  Both factors are powers of
+// two, so the planes are exact multiples of the unscaled ones: the peak
+// search and the log-ratio fit do not depend on them, and the tail divides
+// them out of the reported peak values.
 template <int W, int DT, bool ACC>
-__device__ __forceinline__ float forward_product(float2 (&z)[W], bool real_compat, int t,
+__device__ __forceinline__ float forward_product(float2 (&z)[W], int t,
                                                  LdsT<W> (*lds)[W + 1], float2 (&q)[W],
                                                  float zmax0, float& scale)
 {
-    constexpr int H2 = W / 2;
     // ---- FFT along y (column t), transpose through LDS ----
     fftreg::fft_dif<W, -1>(z);
     wave_sync();                                  // earlier readers of this buffer are done
@@ -144,23 +193,21 @@ __device__ __forceinline__ float forward_product(float2 (&z)[W], bool real_compa
         const int k = (zmax0 == 0.f || zmax == 0.f) ? 0 : max(-60, min(60, 2 * (e0 - e1)));
         scale = ldexpf(1.f, k);
     }
-    const float f = 0.25f * scale;
     // ---- separate the two real spectra and form P = B conj(A) from the
-    // row's spectrum and the mirror row published in LDS ----
+    // row's spectrum and the mirror row published in LDS.  With s = Z +
+    // conj Z(-k) = 2A and d = Z - conj Z(-k) = 2iB, P = -i d conj(s) / 4; the
+    // factor 1/4 (a power of two, like 2^k) is left to the tail, so the
+    // planes come out exactly 4x (4 * 2^k for the second window). ----
     const int m = (W - t) & (W - 1);              // row of -ky
-    const bool compat_row = real_compat && (t == 0 || t == H2);
     auto product = [&](auto K, float2 zr) {       // zr = Z(-k)
         constexpr int kx = decltype(K)::value;
         const float2 zk = z[brev_n<W>(kx)];
-        // s = Z + conj Z(-k) = 2A ; d = Z - conj Z(-k) = 2iB ; P = -i d conj(s) / 4
         const float sr = zk.x + zr.x, si = zk.y - zr.y;
         const float dr = zk.x - zr.x, di = zk.y + zr.y;
-        float pr = f * (di * sr - dr * si);
-        float pi = -f * (dr * sr + di * si);
-        // fft.h:166-183 leaves ky in {0, H/2} and kx = 0 empty
-        if (compat_row || (kx == 0 && real_compat)) { pr = 0.f; pi = 0.f; }
-        if constexpr (ACC) q[kx] = make_float2(q[kx].x - pi, q[kx].y + pr);
-        else q[kx] = make_float2(pr, pi);
+        const float pr = __builtin_fmaf(di, sr, -(dr * si));     // Re 4P
+        const float npi = __builtin_fmaf(dr, sr, di * si);       // -Im 4P
+        if constexpr (ACC) q[kx] = make_float2(__builtin_fmaf(scale, npi, q[kx].x), __builtin_fmaf(scale, pr, q[kx].y));
+        else q[kx] = make_float2(pr, -npi);
     };
     wave_sync();
     lds_exchange<W>(
@@ -199,13 +246,23 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     float scale = 1.f, inv_scale = 1.f;
     const bool compat = p.real_compat != 0;
     float2 z0[W], z1[W];
-    load_column<W, DT>(geo[0], t, z0);
-    if constexpr (PREFETCH) load_column<W, DT>(geo[ROUNDS - 1], t, z1);
-    const float zmax0 = forward_product<W, DT, false>(z0, compat, t, lds, q, 0.f, scale);
+    const TaskFrames frames = task_frames<DT>(geo[0]);
+    load_column<W, DT>(frames, geo[0], t, z0);
+    if constexpr (PREFETCH) load_column<W, DT>(frames, geo[ROUNDS - 1], t, z1);
+    const float zmax0 = forward_product<W, DT, false>(z0, t, lds, q, 0.f, scale);
     if constexpr (ROUNDS == 2) {
-        if constexpr (!PREFETCH) load_column<W, DT>(geo[ROUNDS - 1], t, z1);
-        forward_product<W, DT, true>(z1, compat, t, lds, q, zmax0, scale);
+        if constexpr (!PREFETCH) load_column<W, DT>(frames, geo[ROUNDS - 1], t, z1);
+        forward_product<W, DT, true>(z1, t, lds, q, zmax0, scale);
         inv_scale = 1.f / scale;
+    }
+    if (compat) {
+        // fft.h:166-183 leaves ky in {0, H/2} and kx = 0 empty (in both
+        // packed spectra, so in their sum)
+        const bool empty_row = t == 0 || t == H2;
+        sfor<W>([&](auto K) {
+            constexpr int kx = decltype(K)::value;
+            if (empty_row || kx == 0) q[kx] = make_float2(0.f, 0.f);
+        });
     }
 
     // ---- inverse along x (row ky = t), transpose, inverse along y ----
@@ -225,8 +282,12 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
         constexpr int h = decltype(Hh)::value;
         const float2 v = q[brev_n<W>((h + H2) & (W - 1))];
         s[0][h] = v.x;
-        if constexpr (ROUNDS == 2) s[ROUNDS - 1][h] = v.y * inv_scale;
+        if constexpr (ROUNDS == 2) s[ROUNDS - 1][h] = v.y;
     });
+    // exact factors from these planes back to the correlation's
+    float unscale[ROUNDS];
+    unscale[0] = 0.25f;
+    if constexpr (ROUNDS == 2) unscale[ROUNDS - 1] = 0.25f * inv_scale;
 
     // the planes to LDS (round r, column t, row h; the wave's transposes are
     // done) so that the stencil around each peak is read by address
@@ -242,7 +303,7 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
         });
         wave_sync();
     }
-    finish_windows<W, ROUNDS, PLANES>(p, s, geo, active, lane, t, [&](int r, int row, int col) {
+    finish_windows<W, ROUNDS, PLANES>(p, s, unscale, geo, active, lane, t, [&](int r, int row, int col) {
         return pl[(r * W + col) * (W + 1) + row];
     });
 }

@@ -32,6 +32,11 @@ struct BatchPair {
 // windows per wave task of the kernel for window size win
 constexpr int batch_align(int win) { return (win <= 32 ? 2 : 1) * (64 / win); }
 
+// Largest frame row pitch in bytes.  The FFT kernel addresses a wave task's
+// pixels as 32-bit byte offsets from the task's first window: a task spans
+// at most 129 rows of pitch, plus 63 more for a window's last row.
+constexpr size_t kMaxPitchBytes = size_t(1) << 24;
+
 // One launch processes windows [first, first + n_windows) of a cartesian
 // grid (grid.impl.h:44-58): window g sits at
 // (x_start + (g % nx) * step_x, y_start + (g / nx) * step_y).
