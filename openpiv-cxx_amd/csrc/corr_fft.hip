@@ -169,7 +169,7 @@ __device__ __forceinline__ float forward_product(float2 (&z)[W], int t,
                                                  float zmax0, float& scale)
 {
     // ---- FFT along y (column t), transpose through LDS ----
-    fftreg::fft_dif<W, -1>(z);
+    fftreg::fft_dit<W, -1>(z);
     wave_sync();                                  // earlier readers of this buffer are done
     lds_exchange<W>(
         lds, [&](auto K) { return z[brev_n<W>(decltype(K)::value)]; },
@@ -177,7 +177,7 @@ __device__ __forceinline__ float forward_product(float2 (&z)[W], int t,
         [&](auto X) { return make_int2(t, decltype(X)::value); },
         [&](auto X, float2 v) { z[decltype(X)::value] = v; });
     // ---- FFT along x: lane t holds spectral row ky = t ----
-    fftreg::fft_dif<W, -1>(z);
+    fftreg::fft_dit<W, -1>(z);
     // max |Z| over the window (lanes of the group)
     float zmax = 0.f;
     sfor<W>([&](auto K) {
@@ -266,7 +266,7 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     }
 
     // ---- inverse along x (row ky = t), transpose, inverse along y ----
-    fftreg::fft_dif<W, +1>(q);
+    fftreg::fft_dit<W, +1>(q);
     wave_sync();
     const int xs = (t + H2) & (W - 1);           // plane column that swaps into column t
     lds_exchange<W>(
@@ -274,7 +274,7 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
         [&](auto X) { return make_int2(t, decltype(X)::value); },
         [&](auto K) { return make_int2(decltype(K)::value, xs); },
         [&](auto K, float2 v) { q[decltype(K)::value] = v; });
-    fftreg::fft_dif<W, +1>(q);
+    fftreg::fft_dit<W, +1>(q);
 
     // quadrant-swapped planes, column t: S[h] = c[(h + W/2) mod W]
     float s[ROUNDS][W];

@@ -1,11 +1,11 @@
 // fft_reg.hpp -- fully unrolled in-register FFTs for one lane.
 //
 // A lane holds a whole line of N complex samples in VGPRs (float2 x[N]) and
-// transforms it with a radix-2 decimation-in-frequency network whose loop
-// indices are template constants: twiddles fold into literals, trivial
-// twiddles (1, +-i, (1+-i)/sqrt2) cost no multiplies, and the bit-reversed
-// output order is a compile-time renaming (read element k as x[brev<N>(k)]).
-// No LDS and no cross-lane traffic is involved.
+// transforms it with a radix-2 decimation-in-time network whose loop indices
+// are template constants: twiddles fold into literals, trivial twiddles (1,
+// +-i) cost adds only, every other butterfly is six fused multiply-adds, and
+// the bit-reversed order is a compile-time renaming (read element k as
+// x[brev<N>(k)]).  No LDS and no cross-lane traffic is involved.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -67,7 +67,7 @@ constexpr int brev(int v, int bits)
     return r;
 }
 
-// index of frequency/sample k in the DIF output array of an N-point network
+// index of frequency k in the output array of an N-point network
 template <int N>
 constexpr int brev_n(int k) { return brev(k, ilog2c(N)); }
 
@@ -86,52 +86,60 @@ __device__ __forceinline__ void sfor(F&& f)
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
-// d * exp(SIGN * 2*pi*i * E / N) with the trivial cases folded
+// Radix-2 decimation-in-time butterfly (u, v) <- (u + w v, u - w v), w =
+// exp(SIGN * 2*pi*i * E / N), 0 <= E < N/2, in fused multiply-adds only:
+// with w = c (1 + i s/c) (or s (c/s + i) where |s| > |c|), t = v (1 + i r)
+// takes two FMAs and u +- c t four, against two multiplies, two FMAs and
+// four adds for the plain form.  w = 1 and w = SIGN * i are adds only.
 template <int SIGN, int N, int E>
-__device__ __forceinline__ float2 twiddle(float2 d)
+__device__ __forceinline__ void bfly_dit(float2& u, float2& v)
 {
-    constexpr int e = ((E % N) + N) % N;
-    if constexpr (e == 0) {
-        return d;
-    } else if constexpr (4 * e == N) {          // SIGN * i
-        return SIGN > 0 ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
-    } else if constexpr (2 * e == N) {          // -1
-        return make_float2(-d.x, -d.y);
-    } else if constexpr (4 * e == 3 * N) {      // -SIGN * i
-        return SIGN > 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
-    } else if constexpr (8 * e == N || 8 * e == 3 * N || 8 * e == 5 * N || 8 * e == 7 * N) {
-        // |cos| == |sin| == sqrt(2)/2: two adds and two multiplies
-        constexpr float c = (float)cos2pi(e, N);
-        constexpr float s = (float)(SIGN * sin2pi(e, N));
-        constexpr float m = c > 0 ? c : -c;
-        // (x + iy)(c + is) = (cx - sy) + i(sx + cy)
-        constexpr float cs = c > 0 ? 1.f : -1.f;
-        constexpr float ss = s > 0 ? 1.f : -1.f;
-        return make_float2(m * (cs * d.x - ss * d.y), m * (ss * d.x + cs * d.y));
+    static_assert(E >= 0 && 2 * E < N, "a DIT stage uses exponents 0 <= E < N/2");
+    if constexpr (E == 0) {
+        const float2 a = u;
+        u = cadd(a, v);
+        v = csub(a, v);
+    } else if constexpr (4 * E == N) {           // w = SIGN * i
+        const float2 t = SIGN > 0 ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x);
+        const float2 a = u;
+        u = cadd(a, t);
+        v = csub(a, t);
     } else {
-        constexpr float c = (float)cos2pi(e, N);
-        constexpr float s = (float)(SIGN * sin2pi(e, N));
-        return make_float2(__builtin_fmaf(d.x, c, -(d.y * s)), __builtin_fmaf(d.x, s, d.y * c));
+        constexpr double c = cos2pi(E, N);
+        constexpr double s = SIGN * sin2pi(E, N);
+        constexpr bool cos_major = (c > 0 ? c : -c) >= (s > 0 ? s : -s);
+        constexpr float m = (float)(cos_major ? c : s);
+        constexpr float r = (float)(cos_major ? s / c : c / s);
+        float tx, ty;
+        if constexpr (cos_major) {               // w v = c ((vx - r vy) + i (vy + r vx))
+            tx = __builtin_fmaf(-r, v.y, v.x);
+            ty = __builtin_fmaf(r, v.x, v.y);
+        } else {                                 // w v = s ((r vx - vy) + i (vx + r vy))
+            tx = __builtin_fmaf(r, v.x, -v.y);
+            ty = __builtin_fmaf(r, v.y, v.x);
+        }
+        const float2 a = u;
+        u = make_float2(__builtin_fmaf(m, tx, a.x), __builtin_fmaf(m, ty, a.y));
+        v = make_float2(__builtin_fmaf(-m, tx, a.x), __builtin_fmaf(-m, ty, a.y));
     }
 }
 
-// In-place N-point DIF FFT, SIGN = -1 forward (exp(-2 pi i k n / N)), +1
-// inverse, un-normalised.  Output frequency k is x[brev_n<N>(k)].
+// In-place N-point FFT, SIGN = -1 forward (exp(-2 pi i k n / N)), +1
+// inverse, un-normalised: natural-order input, frequency k at
+// x[brev_n<N>(k)].  A radix-2 DIT network run on the bit-reversed view of the
+// array (logical position p is x[brev(p)]), so the input needs no reordering.
+// Butterflies are bfly_dit's FMA form.
 template <int N, int SIGN>
-__device__ __forceinline__ void fft_dif(float2 (&x)[N])
+__device__ __forceinline__ void fft_dit(float2 (&x)[N])
 {
     constexpr int LOG = ilog2c(N);
     sfor<LOG>([&](auto S) {
-        constexpr int len = N >> decltype(S)::value;
-        constexpr int half = len / 2;
-        sfor<N / len>([&](auto G) {
-            constexpr int start = decltype(G)::value * len;
-            sfor<half>([&](auto K) {
-                constexpr int k = decltype(K)::value;
-                const float2 u = x[start + k];
-                const float2 v = x[start + k + half];
-                x[start + k] = cadd(u, v);
-                x[start + k + half] = twiddle<SIGN, N, k * (N / len)>(csub(u, v));
+        constexpr int m = 2 << decltype(S)::value;       // span of this stage's groups
+        sfor<N / m>([&](auto G) {
+            constexpr int k0 = decltype(G)::value * m;
+            sfor<m / 2>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                bfly_dit<SIGN, N, j * (N / m)>(x[brev_n<N>(k0 + j)], x[brev_n<N>(k0 + j + m / 2)]);
             });
         });
     });

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void fft_transform_kernel(const double2* in, d
         const double2 v = in[base + (size_t)y * W + t];
         z[y] = make_float2((float)v.x, (float)v.y);
     });
-    fftreg::fft_dif<W, SIGN>(z);                  // column t: frequency ky at z[brev(ky)]
+    fftreg::fft_dit<W, SIGN>(z);                  // column t: frequency ky at z[brev(ky)]
     sfor<W>([&](auto K) {
         constexpr int ky = decltype(K)::value;
         lds[ky][t] = z[brev_n<W>(ky)];
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void fft_transform_kernel(const double2* in, d
         constexpr int x = decltype(X)::value;
         z[x] = lds[t][x];                        // row ky = t
     });
-    fftreg::fft_dif<W, SIGN>(z);                  // frequency kx at z[brev(kx)]
+    fftreg::fft_dit<W, SIGN>(z);                  // frequency kx at z[brev(kx)]
     wave_sync_t();                                // the row reads are done
     sfor<W>([&](auto K) {
         constexpr int kx = decltype(K)::value;
