@@ -113,108 +113,116 @@ template <int W>
 using LdsT = std::conditional_t<split_lds<W>(), float, float2>;
 
 // Complex exchange through the lane group's LDS slice: this lane's values
-// src(i) go to lds[wi(i)], then dst(j, value) receives lds[ri(j)] (i, j =
-// 0..W-1, compile-time).  The caller has synchronised earlier readers of the
-// slice; src is fully read before dst is called.
-template <int W, class Src, class Wi, class Ri, class Dst>
+// src(i) go to lds[wi(i)] (i = 0..NW-1), then dst(j, value) receives
+// lds[ri(j)] (j = 0..NR-1); i and j are compile-time.  The caller has
+// synchronised earlier readers of the slice; src is fully read before dst is
+// called.
+template <int W, int NW, int NR, class Src, class Wi, class Ri, class Dst>
 __device__ __forceinline__ void lds_exchange(LdsT<W> (*lds)[W + 1], Src&& src, Wi&& wi, Ri&& ri, Dst&& dst)
 {
     if constexpr (!split_lds<W>()) {
-        sfor<W>([&](auto I) {
+        sfor<NW>([&](auto I) {
             const int2 a = wi(I);
             lds[a.x][a.y] = src(I);
         });
         wave_sync();
-        sfor<W>([&](auto J) {
+        sfor<NR>([&](auto J) {
             const int2 a = ri(J);
             dst(J, lds[a.x][a.y]);
         });
     } else {
-        float re[W];
-        sfor<W>([&](auto I) {
+        float re[NR];
+        sfor<NW>([&](auto I) {
             const int2 a = wi(I);
             lds[a.x][a.y] = src(I).x;
         });
         wave_sync();
-        sfor<W>([&](auto J) {
+        sfor<NR>([&](auto J) {
             const int2 a = ri(J);
             re[decltype(J)::value] = lds[a.x][a.y];
         });
         wave_sync();
-        sfor<W>([&](auto I) {
+        sfor<NW>([&](auto I) {
             const int2 a = wi(I);
             lds[a.x][a.y] = src(I).y;
         });
         wave_sync();
-        sfor<W>([&](auto J) {
+        sfor<NR>([&](auto J) {
             const int2 a = ri(J);
             dst(J, make_float2(re[decltype(J)::value], lds[a.x][a.y]));
         });
     }
 }
 
-// Transform the window's column t (z, from load_column; consumed) and leave
-// in q (natural order, lane t = spectral row ky) 4 P, P = B conj(A) the
-// cross-power spectrum of this lane's row -- or, with ACC, add i 2^k 4 P to
-// q.  Returns the window's max |Z| component (Z = FFT(a + i b)), which
-// bounds |P| <= max|Z|^2.  The power of two 2^k equalises that bound between
-// the two packed spectra (zmax0 is the first window's), so each plane's FP32
-// rounding stays relative to its own magnitude.  Both factors are powers of
-// two, so the planes are exact multiples of the unscaled ones: the peak
-// search and the log-ratio fit do not depend on them, and the tail divides
-// them out of the reported peak values.
-template <int W, int DT, bool ACC>
-__device__ __forceinline__ float forward_product(float2 (&z)[W], int t,
-                                                 LdsT<W> (*lds)[W + 1], float2 (&q)[W],
-                                                 float zmax0, float& scale)
+// Transform the window's column t (z, from load_column) in place: lane t
+// then holds spectral row ky = t of Z = FFT(a + i b), Z(t, kx) at
+// z[brev_n<W>(kx)].  Returns the window's max |Z| component, which bounds the
+// cross-power spectrum: |P| <= max|Z|^2.
+template <int W>
+__device__ __forceinline__ float forward_spectrum(float2 (&z)[W], int t, LdsT<W> (*lds)[W + 1])
 {
     // ---- FFT along y (column t), transpose through LDS ----
     fftreg::fft_dit<W, -1>(z);
-    wave_sync();                                  // earlier readers of this buffer are done
-    lds_exchange<W>(
+    wave_sync();                                  // earlier readers of this slice are done
+    lds_exchange<W, W, W>(
         lds, [&](auto K) { return z[brev_n<W>(decltype(K)::value)]; },
         [&](auto K) { return make_int2(decltype(K)::value, t); },
         [&](auto X) { return make_int2(t, decltype(X)::value); },
         [&](auto X, float2 v) { z[decltype(X)::value] = v; });
     // ---- FFT along x: lane t holds spectral row ky = t ----
     fftreg::fft_dit<W, -1>(z);
-    // max |Z| over the window (lanes of the group)
     float zmax = 0.f;
     sfor<W>([&](auto K) {
         constexpr int k = decltype(K)::value;
         zmax = fmaxf(zmax, fmaxf(fabsf(z[k].x), fabsf(z[k].y)));
     });
-    zmax = group_max<W>(zmax);
-    scale = 1.f;
-    if constexpr (ACC) {
-        int e0, e1;
-        frexpf(zmax0, &e0);
-        frexpf(zmax, &e1);
-        const int k = (zmax0 == 0.f || zmax == 0.f) ? 0 : max(-60, min(60, 2 * (e0 - e1)));
-        scale = ldexpf(1.f, k);
-    }
-    // ---- separate the two real spectra and form P = B conj(A) from the
-    // row's spectrum and the mirror row published in LDS.  With s = Z +
-    // conj Z(-k) = 2A and d = Z - conj Z(-k) = 2iB, P = -i d conj(s) / 4; the
-    // factor 1/4 (a power of two, like 2^k) is left to the tail, so the
-    // planes come out exactly 4x (4 * 2^k for the second window). ----
+    return group_max<W>(zmax);
+}
+
+// Half of this lane's row of 4 P, P = B conj(A) the cross-power spectrum:
+// h[kx] = 4 P(t, kx) for kx = 0..W/2.  With s = Z + conj Z(-k) = 2A and
+// d = Z - conj Z(-k) = 2iB, P = -i d conj(s) / 4; the mirror row's Z(-t, -kx)
+// comes through LDS, and only its columns 0 and W/2..W-1 are needed.  A and
+// B are spectra of real windows, so P(-k) = conj P(k): the row's other half
+// is the conjugate of the mirror lane's first half.  The factor 4 is a
+// power of two and stays in the planes.
+template <int W>
+__device__ __forceinline__ void half_product(const float2 (&z)[W], int t, LdsT<W> (*lds)[W + 1],
+                                             float2 (&h)[W / 2 + 1])
+{
+    constexpr int H2 = W / 2;
     const int m = (W - t) & (W - 1);              // row of -ky
-    auto product = [&](auto K, float2 zr) {       // zr = Z(-k)
-        constexpr int kx = decltype(K)::value;
-        const float2 zk = z[brev_n<W>(kx)];
-        const float sr = zk.x + zr.x, si = zk.y - zr.y;
-        const float dr = zk.x - zr.x, di = zk.y + zr.y;
-        const float pr = __builtin_fmaf(di, sr, -(dr * si));     // Re 4P
-        const float npi = __builtin_fmaf(dr, sr, di * si);       // -Im 4P
-        if constexpr (ACC) q[kx] = make_float2(__builtin_fmaf(scale, npi, q[kx].x), __builtin_fmaf(scale, pr, q[kx].y));
-        else q[kx] = make_float2(pr, -npi);
-    };
-    wave_sync();
-    lds_exchange<W>(
-        lds, [&](auto K) { return z[brev_n<W>(decltype(K)::value)]; },
-        [&](auto K) { return make_int2(t, decltype(K)::value); },
-        [&](auto K) { return make_int2(m, (W - decltype(K)::value) & (W - 1)); }, product);
-    return zmax;
+    wave_sync();                                  // earlier readers of this slice are done
+    lds_exchange<W, H2 + 1, H2 + 1>(
+        lds,
+        [&](auto I) {
+            constexpr int i = decltype(I)::value;
+            return z[brev_n<W>(i == 0 ? 0 : H2 - 1 + i)];
+        },
+        [&](auto I) {
+            constexpr int i = decltype(I)::value;
+            return make_int2(t, i == 0 ? 0 : H2 - 1 + i);
+        },
+        [&](auto K) { return make_int2(m, (W - decltype(K)::value) & (W - 1)); },
+        [&](auto K, float2 zr) {                  // zr = Z(-k)
+            constexpr int kx = decltype(K)::value;
+            const float2 zk = z[brev_n<W>(kx)];
+            const float sr = zk.x + zr.x, si = zk.y - zr.y;
+            const float dr = zk.x - zr.x, di = zk.y + zr.y;
+            h[kx] = make_float2(__builtin_fmaf(di, sr, -(dr * si)), -__builtin_fmaf(dr, sr, di * si));
+        });
+}
+
+// The power of two 2^k that equalises the bounds max|Z|^2 of two packed
+// spectra, so that each plane's FP32 rounding stays relative to its own
+// magnitude; multiplying by it is exact.
+__device__ __forceinline__ float balance_scale(float zmax0, float zmax1)
+{
+    int e0, e1;
+    frexpf(zmax0, &e0);
+    frexpf(zmax1, &e1);
+    const int k = (zmax0 == 0.f || zmax1 == 0.f) ? 0 : max(-60, min(60, 2 * (e0 - e1)));
+    return ldexpf(1.f, k);
 }
 
 // One wave task: ROUNDS x (64 / W) windows, windows
@@ -237,24 +245,52 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     });
 
     // ---- forward transforms; with two rounds the spectra pack as
-    // Q = P0 + i P1, whose inverse is c0 + i c1 (both planes are real).
-    // At W = 32 (two waves per SIMD) the second window's columns are loaded
-    // before the first window is transformed, so their latency hides behind
-    // that work. ----
+    // Q = P0 + i 2^k P1, whose inverse is c0 + i 2^k c1 (both planes are
+    // real).  At W = 32 (two waves per SIMD) the second window's columns are
+    // loaded before the first window is transformed, so their latency hides
+    // behind that work. ----
     constexpr bool PREFETCH = ROUNDS == 2 && W == 32;
     float2 q[W];
-    float scale = 1.f, inv_scale = 1.f;
+    float inv_scale = 1.f;
     const bool compat = p.real_compat != 0;
+    const int m = (W - t) & (W - 1);             // row of -ky
     float2 z0[W], z1[W];
     const TaskFrames frames = task_frames<DT>(geo[0]);
     load_column<W, DT>(frames, geo[0], t, z0);
     if constexpr (PREFETCH) load_column<W, DT>(frames, geo[ROUNDS - 1], t, z1);
-    const float zmax0 = forward_product<W, DT, false>(z0, t, lds, q, 0.f, scale);
+    const float zmax0 = forward_spectrum<W>(z0, t, lds);
+    float2 h0[H2 + 1];
+    half_product<W>(z0, t, lds, h0);
+    // q[kx] for kx = 0..W/2, and r[kx] = the mirror lane's q[W - kx] for
+    // kx = 1..W/2-1: Q(-t, -kx) = conj P0(t, kx) + i 2^k conj P1(t, kx)
+    float2 r[H2 + 1];
     if constexpr (ROUNDS == 2) {
         if constexpr (!PREFETCH) load_column<W, DT>(frames, geo[ROUNDS - 1], t, z1);
-        forward_product<W, DT, true>(z1, t, lds, q, zmax0, scale);
+        const float zmax1 = forward_spectrum<W>(z1, t, lds);
+        const float scale = balance_scale(zmax0, zmax1);
         inv_scale = 1.f / scale;
+        float2 h1[H2 + 1];
+        half_product<W>(z1, t, lds, h1);
+        sfor<H2 + 1>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            const float2 a = h0[k], c = h1[k];
+            q[k] = make_float2(__builtin_fmaf(-scale, c.y, a.x), __builtin_fmaf(scale, c.x, a.y));
+            if constexpr (k >= 1 && k < H2)
+                r[k] = make_float2(__builtin_fmaf(scale, c.y, a.x), __builtin_fmaf(scale, c.x, -a.y));
+        });
+    } else {
+        sfor<H2 + 1>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            q[k] = h0[k];
+            if constexpr (k >= 1 && k < H2) r[k] = make_float2(h0[k].x, -h0[k].y);
+        });
     }
+    wave_sync();                                 // half_product's readers of the slice are done
+    lds_exchange<W, H2 - 1, H2 - 1>(
+        lds, [&](auto I) { return r[decltype(I)::value + 1]; },
+        [&](auto I) { return make_int2(t, decltype(I)::value + 1); },
+        [&](auto J) { return make_int2(m, decltype(J)::value + 1); },
+        [&](auto J, float2 v) { q[W - 1 - decltype(J)::value] = v; });
     if (compat) {
         // fft.h:166-183 leaves ky in {0, H/2} and kx = 0 empty (in both
         // packed spectra, so in their sum)
@@ -269,7 +305,7 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     fftreg::fft_dit<W, +1>(q);
     wave_sync();
     const int xs = (t + H2) & (W - 1);           // plane column that swaps into column t
-    lds_exchange<W>(
+    lds_exchange<W, W, W>(
         lds, [&](auto X) { return q[brev_n<W>(decltype(X)::value)]; },
         [&](auto X) { return make_int2(t, decltype(X)::value); },
         [&](auto K) { return make_int2(decltype(K)::value, xs); },
