@@ -56,19 +56,25 @@ __device__ __forceinline__ float load_px(const void* base, size_t idx)
     else return (float)((gptr<double>)base)[idx];
 }
 
-// d(num/den)/d(l, c, r) with num = ln l - ln r, den = 2 (ln l + ln r - 2 ln c)
-__device__ __forceinline__ void gauss3_grad(double l, double c, double r, double& gl, double& gc, double& gr)
+// 1 / x to FP64 precision: the hardware reciprocal and two Newton steps
+__device__ __forceinline__ double rcp64(double x)
 {
-    const double L = log(l), R = log(r), Cc = log(c);
-    const double num = L - R, den = 2.0 * (L + R - 2.0 * Cc);
-    if (den == 0.0) {
-        gl = gc = gr = 0.0;
-        return;
-    }
-    const double d2 = den * den;
-    gl = (den - 2.0 * num) / d2 / l;
-    gr = (-den - 2.0 * num) / d2 / r;
-    gc = 4.0 * num / d2 / c;
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
+// d(num/den)/d(l, c, r) with num = ln l - ln r, den = 2 (ln l + ln r - 2 ln c)
+// (image_utils.impl.h:86-94), from the values and their logs; zero where den
+// is zero (the fit's sub-pixel term is then 0)
+__device__ __forceinline__ void gauss3_grad(double l, double c, double r, double lnl, double lnc, double lnr,
+                                            double& gl, double& gc, double& gr)
+{
+    const double num = lnl - lnr, den = 2.0 * (lnl + lnr - 2.0 * lnc);
+    const double q = den == 0.0 ? 0.0 : rcp64(den * den);
+    gl = (den - 2.0 * num) * q * rcp64(l);
+    gr = (-den - 2.0 * num) * q * rcp64(r);
+    gc = 4.0 * num * q * rcp64(c);
 }
 
 } // namespace
@@ -101,22 +107,27 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjParams p)
     const opiv_window_result fw = p.fwd[i];
     const bool valid = active && !(fw.flags & (OPIV_FLAG_FEWER_THAN_2_PEAKS | OPIV_FLAG_NONFINITE_FIT));
 
-    // ---- both windows to LDS, column t of frame A kept in registers ----
-    float acol[W];
-    {
-        const size_t idx = (size_t)y0 * p.pitch + (size_t)(x0 + t);
-        sfor<W>([&](auto Y) {
-            constexpr int y = decltype(Y)::value;
-            acol[y] = load_px<DT>(p.a, idx + (size_t)y * p.pitch);
-            if constexpr (VARIANCE) la[y][t] = acol[y];
-            lb[y][t] = load_px<DT>(p.b, idx + (size_t)y * p.pitch);
-        });
-    }
-    wave_sync_adj();
-
     // peak shift in correlation coordinates: swapped (h, w) holds
     // C[(h + W/2) % W][(w + W/2) % W]
     const int sx0 = (fw.px + H2) & M, sy0 = (fw.py + H2) & M;
+
+    // ---- both windows to LDS, column t of frame A kept in registers.  The
+    // LDS copies are stored rotated by the peak's row shift, frame B as
+    // lb[j][x] = b[(j + sy0 - 1) mod W][x] and frame A (read by the variance
+    // gradients) as la[j][x] = a[(j - sy0 - 1) mod W][x], so every tap below
+    // reads its rows at compile-time offsets. ----
+    float acol[W];
+    {
+        const size_t idx = (size_t)y0 * p.pitch + (size_t)(x0 + t);
+        const int rb = (1 - sy0) & M, ra = (sy0 + 1) & M;   // LDS rows of window row 0
+        sfor<W>([&](auto Y) {
+            constexpr int y = decltype(Y)::value;
+            acol[y] = load_px<DT>(p.a, idx + (size_t)y * p.pitch);
+            if constexpr (VARIANCE) la[(ra + y) & M][t] = acol[y];
+            lb[(rb + y) & M][t] = load_px<DT>(p.b, idx + (size_t)y * p.pitch);
+        });
+    }
+    wave_sync_adj();
 
     // taps k = l, c, r, up, down at shifts s_k = (sy0 + dy, sx0 + dx) with
     // (dy, dx) = (0,-1), (0,0), (0,1), (-1,0), (1,0).  The taps touch three
@@ -134,7 +145,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjParams p)
         float bc[W + 2];                                // bc[j] = b[(j + sy0 - 1) % W][col]
         sfor<W + 2>([&](auto J) {
             constexpr int j = decltype(J)::value;
-            bc[j] = lb[(j + sy0 - 1) & M][col];
+            bc[j] = lb[j & M][col];
         });
         sfor<5>([&](auto K) {
             constexpr int k = decltype(K)::value;
@@ -156,18 +167,36 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjParams p)
         });
     }
 
-    // ---- du/dC_k and dv/dC_k (u = W/2 - px - dx, v = W/2 - py - dy) ----
+    // ---- du/dC_k and dv/dC_k (u = W/2 - px - dx, v = W/2 - py - dy).  The
+    // five logs are taken one per lane (lane k of the group takes ln C_k)
+    // and shared through the group. ----
+    double lnk[5];
+    {
+        double mine = ck[0];
+        sfor<4>([&](auto K) {
+            constexpr int k = decltype(K)::value + 1;
+            mine = t == k ? ck[k] : mine;
+        });
+        const double lg = log(mine);
+        const int gbase = lane - t;
+        sfor<5>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            lnk[k] = __shfl(lg, gbase + k, 64);
+        });
+    }
     double gu[5] = {0, 0, 0, 0, 0}, gv[5] = {0, 0, 0, 0, 0};
-    if (valid) {
+    {
         double gl, gcx, gr, gt, gcy, gbt;
-        gauss3_grad(ck[0], ck[1], ck[2], gl, gcx, gr);
-        gauss3_grad(ck[3], ck[1], ck[4], gt, gcy, gbt);
-        gu[0] = -gl;
-        gu[1] = -gcx;
-        gu[2] = -gr;
-        gv[1] = -gcy;
-        gv[3] = -gt;
-        gv[4] = -gbt;
+        gauss3_grad(ck[0], ck[1], ck[2], lnk[0], lnk[1], lnk[2], gl, gcx, gr);
+        gauss3_grad(ck[3], ck[1], ck[4], lnk[3], lnk[1], lnk[4], gt, gcy, gbt);
+        if (valid) {
+            gu[0] = -gl;
+            gu[1] = -gcx;
+            gu[2] = -gr;
+            gv[1] = -gcy;
+            gv[3] = -gt;
+            gv[4] = -gbt;
+        }
     }
     float fu[5], fv[5];
     {
@@ -213,14 +242,14 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjParams p)
         sfor<3>([&](auto D) {
             constexpr int d = decltype(D)::value;
             // frame B column t + sx0 + dx rotated by sy0 - 1, or frame A
-            // column t - sx0 - dx rotated by -sy0 - 1
+            // column t - sx0 - dx rotated by -sy0 - 1 (the LDS copies are
+            // stored with these rotations)
             const int col = fr == 0 ? ((t + sx0 + d - 1) & M) : ((t - sx0 - (d - 1)) & M);
-            const int rot = fr == 0 ? (sy0 - 1) : (-sy0 - 1);
             float (*src)[W + 1] = fr == 0 ? lb : la;
             float cc[W + 2];
             sfor<W + 2>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                cc[j] = src[(j + rot) & M][col];
+                cc[j] = src[j & M][col];
             });
             sfor<5>([&](auto K) {
                 constexpr int k = decltype(K)::value;
