@@ -368,7 +368,8 @@ __global__ __launch_bounds__(1024) void adjoint_gather_tiled_kernel(AdjParams p)
         wc[i] = p.coef[(size_t)(iya + wy) * p.nx + ixa + wx];
     }
 
-    const int c = (int)(threadIdx.x & 63), rg = (int)(threadIdx.x >> 6);
+    // rg is the wave index, so every row test below is wave-uniform
+    const int c = (int)(threadIdx.x & 63), rg = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int x = tx0 + c, ya = ty0 + rg * R;
     const int dx = x - p.x_start;
     const int ix_lo = dx < W ? 0 : (dx - W) / p.step_x + 1;
@@ -417,21 +418,29 @@ __global__ __launch_bounds__(1024) void adjoint_gather_tiled_kernel(AdjParams p)
                 const int cx = fr == 0 ? ((lx + sx0) & M) : ((lx - sx0) & M);
                 const int cl = (cx + (fr == 0 ? -1 : 1)) & M, cr = (cx + (fr == 0 ? 1 : -1)) & M;
                 const int ob = (y0 - ry0) * RS + (x0 - rx0);
+                // tap rows around the thread's rows: row e (e = 0..R+1) of the
+                // window is (ly0 + e - 1 + sy0) mod W (frame B) or with -sy0
+                // (frame A); the thread's row i reads l, c, r on row i + 1 and
+                // up/down on rows i and i + 2 (swapped for frame A, whose
+                // shift is negated)
+                const int rb = ly0 - 1 + (fr == 0 ? sy0 : -sy0);
+                int rowc[R + 2];
+                float vc[R + 2];
+                sfor<R + 2>([&](auto E) {
+                    constexpr int e = decltype(E)::value;
+                    rowc[e] = ob + ((rb + e) & M) * RS;
+                    vc[e] = reg[rowc[e] + cx];
+                });
                 sfor<R>([&](auto I) {
                     constexpr int i = decltype(I)::value;
                     const int ly = ly0 + i;
                     if (ly >= 0 && ly < W) {
-                        const int ry = fr == 0 ? ((ly + sy0) & M) : ((ly - sy0) & M);
-                        const int rm = (ry - 1) & M, rp = (ry + 1) & M;
-                        const int o = ob + ry * RS;
-                        // taps l, c, r, up, down; for frame A (grad_b) the shift is
-                        // negated, which swaps l <-> r and up <-> down
                         float t = g[i];
-                        t = fmaf(cf.f[0], reg[o + cl], t);
-                        t = fmaf(cf.f[1], reg[o + cx], t);
-                        t = fmaf(cf.f[2], reg[o + cr], t);
-                        t = fmaf(cf.f[3], reg[ob + (fr == 0 ? rm : rp) * RS + cx], t);
-                        t = fmaf(cf.f[4], reg[ob + (fr == 0 ? rp : rm) * RS + cx], t);
+                        t = fmaf(cf.f[0], reg[rowc[i + 1] + cl], t);
+                        t = fmaf(cf.f[1], vc[i + 1], t);
+                        t = fmaf(cf.f[2], reg[rowc[i + 1] + cr], t);
+                        t = fmaf(cf.f[3], vc[fr == 0 ? i : i + 2], t);
+                        t = fmaf(cf.f[4], vc[fr == 0 ? i + 2 : i], t);
                         g[i] = t;
                     }
                 });
