@@ -57,8 +57,7 @@ typedef enum opiv_variant {
     OPIV_DIRECT_BF16_SPLIT = 2,
     /* the faster of OPIV_CIRCULAR_FFT_FP32 and OPIV_DIRECT_BF16_SPLIT for
      * the plan's window size and pixel type as measured on MI355X (same
-     * semantics): the direct path for 8-bit frames at 16x16, the FFT path
-     * otherwise */
+     * semantics); at present the FFT path for every size and type */
     OPIV_AUTO = 3
 } opiv_variant;
 

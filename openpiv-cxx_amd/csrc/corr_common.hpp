@@ -239,12 +239,15 @@ __device__ __forceinline__ float group_max(float v)
 // depend on it, and it is multiplied back (exactly) into the reported peak
 // values and planes.  fetch(r, row, col) reads the value of round r's plane
 // at (row, col) from LDS (any lane may read any position of its group's
-// planes); it supplies the stencil around each peak.
-template <int W, int ROUNDS, bool PLANES, class Fetch>
+// planes); it supplies the stencil around each peak.  With SEQ the LDS
+// holds one plane at a time: stage(r) (a compile-time round) puts plane r
+// there, and each lane reads the values it needs of one round before the
+// next is staged; otherwise every plane is readable on entry.
+template <int W, int ROUNDS, bool PLANES, bool SEQ, class Stage, class Fetch>
 __device__ __forceinline__ void finish_windows(const CorrParams& p, const float (&s)[ROUNDS][W],
                                                const float (&unscale)[ROUNDS],
                                                const WinGeom (&geo)[ROUNDS], const bool (&active)[ROUNDS],
-                                               int lane, int t, Fetch&& fetch)
+                                               int lane, int t, Stage&& stage, Fetch&& fetch)
 {
     const int gbase = lane - t;
     if constexpr (PLANES) {
@@ -295,12 +298,41 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
         // value j of round r sits on lane j * ROUNDS + r of the group ----
         constexpr int NV = 5 * ROUNDS;
         constexpr int NP = (NV + W - 1) / W;
+        float sv[NP];          // the stencil value whose log this lane takes
+        float nb[4];           // lanes t < ROUNDS: the four neighbours of round t's peak
+        if constexpr (SEQ) {
+            sfor<ROUNDS>([&](auto R) {
+                constexpr int r = decltype(R)::value;
+                stage(R);
+                auto at = [&](int j) {
+                    const int cx = px[r] + (j == 0 ? -1 : (j == 2 ? 1 : 0));
+                    const int cy = py[r] + (j == 3 ? -1 : (j == 4 ? 1 : 0));
+                    return fetch(r, cy, cx);
+                };
+                sfor<NP>([&](auto K) {
+                    constexpr int k = decltype(K)::value;
+                    const int L = t + k * W;
+                    if (L % ROUNDS == r) sv[k] = at(L < NV ? L / ROUNDS : 1);
+                });
+                if (t == r) {
+                    nb[0] = at(0);
+                    nb[1] = at(2);
+                    nb[2] = at(3);
+                    nb[3] = at(4);
+                }
+            });
+        } else {
+            sfor<NP>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                const int L = t + k * W;
+                sv[k] = stencil(L % ROUNDS, L < NV ? L / ROUNDS : 1);
+            });
+        }
         double lg[NP];
         sfor<NP>([&](auto K) {
             constexpr int k = decltype(K)::value;
             const int L = t + k * W;
-            const float v = stencil(L % ROUNDS, L < NV ? L / ROUNDS : 1);
-            lg[k] = log((double)(L < NV ? v : 1.f));
+            lg[k] = log((double)(L < NV ? sv[k] : 1.f));
         });
 
         // ---- lane t < ROUNDS of the group fits window t ----
@@ -348,7 +380,13 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
         if (t >= ROUNDS || !pick(act)) return;
         // the four stencil neighbours of the peak (near-tie test)
         const float us = pick(unscale);
-        const float nmax = us * fmaxf(fmaxf(stencil(rr, 0), stencil(rr, 2)), fmaxf(stencil(rr, 3), stencil(rr, 4)));
+        if constexpr (!SEQ) {
+            nb[0] = stencil(rr, 0);
+            nb[1] = stencil(rr, 2);
+            nb[2] = stencil(rr, 3);
+            nb[3] = stencil(rr, 4);
+        }
+        const float nmax = us * fmaxf(fmaxf(nb[0], nb[1]), fmaxf(nb[2], nb[3]));
         const float fv1 = us * pick(v1), fv2 = us * pick(v2);
         const int fpx = pick(px), fpy = pick(py);
         const int x0 = pick(gx0), y0 = pick(gy0);

@@ -311,7 +311,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
     // unnormalised inverse FFT), an exact power of two that the tail restores
     float unscale[ROUNDS];
     sfor<ROUNDS>([&](auto R) { unscale[decltype(R)::value] = (float)(W * W); });
-    finish_windows<W, ROUNDS, PLANES>(p, s, unscale, geo, active, lane, t, [&](int r, int row, int col) {
+    finish_windows<W, ROUNDS, PLANES, false>(p, s, unscale, geo, active, lane, t, [](auto) {}, [&](int r, int row, int col) {
         return lds_p[r * WPW + slot][row][col];
     });
 }

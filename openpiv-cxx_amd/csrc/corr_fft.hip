@@ -104,11 +104,12 @@ __device__ __forceinline__ void load_column(const TaskFrames& f, const WinGeom& 
     });
 }
 
-// At 64 x 64 the lane group's LDS slice holds floats, and every complex
-// exchange goes through it in two passes (real parts, then imaginary parts):
-// half the LDS, so two waves fit per SIMD.
+// From 32 x 32 up the lane group's LDS slice holds floats, and every
+// complex exchange goes through it in two passes (real parts, then imaginary
+// parts): half the LDS, so that three waves (32 x 32) or two (64 x 64) fit
+// per SIMD.
 template <int W>
-constexpr bool split_lds() { return W == 64; }
+constexpr bool split_lds() { return W >= 32; }
 template <int W>
 using LdsT = std::conditional_t<split_lds<W>(), float, float2>;
 
@@ -246,10 +247,10 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
 
     // ---- forward transforms; with two rounds the spectra pack as
     // Q = P0 + i 2^k P1, whose inverse is c0 + i 2^k c1 (both planes are
-    // real).  At W = 32 (two waves per SIMD) the second window's columns are
-    // loaded before the first window is transformed, so their latency hides
-    // behind that work. ----
-    constexpr bool PREFETCH = ROUNDS == 2 && W == 32;
+    // real).  PREFETCH would load the second window's columns before the
+    // first window is transformed; it costs the registers of a third wave
+    // per SIMD at 32 x 32, and is off. ----
+    constexpr bool PREFETCH = false;
     float2 q[W];
     float inv_scale = 1.f;
     const bool compat = p.real_compat != 0;
@@ -326,9 +327,20 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     if constexpr (ROUNDS == 2) unscale[ROUNDS - 1] = 0.25f * inv_scale;
 
     // the planes to LDS (round r, column t, row h; the wave's transposes are
-    // done) so that the stencil around each peak is read by address
+    // done) so that the stencil around each peak is read by address.  The
+    // split slice holds one plane: then the tail stages them one at a time.
+    constexpr bool SEQ = split_lds<W>() && ROUNDS > 1;
     float* pl = reinterpret_cast<float*>(&lds[0][0]);
-    if constexpr (!PLANES) {
+    auto stage = [&](auto R) {
+        constexpr int r = decltype(R)::value;
+        wave_sync();                             // earlier readers of the slice are done
+        sfor<W>([&](auto Hh) {
+            constexpr int h = decltype(Hh)::value;
+            pl[((SEQ ? 0 : r) * W + t) * (W + 1) + h] = s[r][h];
+        });
+        wave_sync();
+    };
+    if constexpr (!PLANES && !SEQ) {
         wave_sync();
         sfor<ROUNDS>([&](auto R) {
             constexpr int r = decltype(R)::value;
@@ -339,8 +351,8 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
         });
         wave_sync();
     }
-    finish_windows<W, ROUNDS, PLANES>(p, s, unscale, geo, active, lane, t, [&](int r, int row, int col) {
-        return pl[(r * W + col) * (W + 1) + row];
+    finish_windows<W, ROUNDS, PLANES, SEQ>(p, s, unscale, geo, active, lane, t, stage, [&](int r, int row, int col) {
+        return pl[((SEQ ? 0 : r) * W + col) * (W + 1) + row];
     });
 }
 
@@ -348,7 +360,7 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
 // register-resident lines (nearly) without spilling; batch kernels carry
 // per-lane frame pointers and get a little more room at W = 16
 template <int W, bool BATCH>
-constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (BATCH ? 3 : 4) : 2; }
+constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (BATCH ? 3 : 4) : W == 32 ? 3 : 2; }
 
 // BATCH reads each window's pair from the device table (per-lane frame
 // pointers) and loops over wave tasks, so the launch may be persistent; a
