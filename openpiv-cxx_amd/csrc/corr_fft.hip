@@ -104,12 +104,13 @@ __device__ __forceinline__ void load_column(const TaskFrames& f, const WinGeom& 
     });
 }
 
-// From 32 x 32 up the lane group's LDS slice holds floats, and every
+// From 16 x 16 up the lane group's LDS slice holds floats, and every
 // complex exchange goes through it in two passes (real parts, then imaginary
-// parts): half the LDS, so that three waves (32 x 32) or two (64 x 64) fit
-// per SIMD.
+// parts): half the LDS, so that five waves (16 x 16), three (32 x 32) or two
+// (64 x 64) fit per SIMD.  The kernel is bound by latency (LDS round trips,
+// loads), which more resident waves hide.
 template <int W>
-constexpr bool split_lds() { return W >= 32; }
+constexpr bool split_lds() { return W >= 16; }
 template <int W>
 using LdsT = std::conditional_t<split_lds<W>(), float, float2>;
 
@@ -356,11 +357,11 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     });
 }
 
-// occupancy target per window size: the VGPR budget that keeps the
-// register-resident lines (nearly) without spilling; batch kernels carry
-// per-lane frame pointers and get a little more room at W = 16
+// occupancy target per window size: the most waves whose VGPR budget keeps
+// the register-resident lines without spilling; batch kernels carry per-lane
+// frame pointers and get a little more room at W = 16
 template <int W, bool BATCH>
-constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (BATCH ? 3 : 4) : W == 32 ? 3 : 2; }
+constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (BATCH ? 4 : 5) : W == 32 ? 3 : 2; }
 
 // BATCH reads each window's pair from the device table (per-lane frame
 // pointers) and loops over wave tasks, so the launch may be persistent; a
