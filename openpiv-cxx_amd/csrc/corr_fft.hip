@@ -412,7 +412,11 @@ static hipError_t launch_w(const CorrParams& p, bool planes, int nt, int blocks,
 {
     if (p.pairs == nullptr) {
         if (nt != 256) return hipErrorInvalidValue;
-        return launch_wn<W, DT, 256, false>(p, planes, 0, stream);   // one task per wave
+        // one task per wave; at 64 x 64 (LDS-bound at two waves per SIMD) one
+        // wave per block, so that a finished wave frees its LDS without
+        // waiting for the block's others (2.5% faster; 2% slower at 32 x 32)
+        if constexpr (W == 64) return launch_wn<W, DT, 64, false>(p, planes, 0, stream);
+        else return launch_wn<W, DT, 256, false>(p, planes, 0, stream);
     }
     if (planes) return hipErrorInvalidValue;
     if (nt == 256) return launch_wn<W, DT, 256, true>(p, false, blocks, stream);
