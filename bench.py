@@ -310,8 +310,10 @@ def load_traffic(key: str):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # defaults: ~10 ms of warm-up launches, then ~60 ms timed, so the clock has
+    # settled and launch-to-launch jitter averages out
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--win", type=int, default=32)
     ap.add_argument("--step", type=int, default=0, help="grid step (default: half the window, 50%% overlap)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
