@@ -30,6 +30,7 @@
 
 #include "adjoint.hpp"
 #include "fft_reg.hpp"
+#include "xcd.hpp"
 
 namespace opiv {
 
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjParams p)
     float (*lb)[W + 1] = lds_all[wave][slot][0];
     float (*la)[W + 1] = lds_all[wave][slot][NF - 1];
 
-    const int local = (blockIdx.x * WAVES + wave) * WPW + slot;
+    const int local = (xcd_block((int)blockIdx.x, (int)gridDim.x) * WAVES + wave) * WPW + slot;
     const bool active = local < p.n_windows;
     const int i = active ? local : p.n_windows - 1;
     const int ix = i % p.nx, iy = i / p.nx;

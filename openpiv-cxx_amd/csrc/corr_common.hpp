@@ -12,6 +12,7 @@
 #include <cstdint>
 
 #include "corr_fft.hpp"
+#include "xcd.hpp"
 #include "fft_reg.hpp"
 
 namespace opiv {

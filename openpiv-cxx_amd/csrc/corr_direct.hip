@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int task = blockIdx.x * WAVES + wave;
+    const int task = xcd_block((int)blockIdx.x, (int)gridDim.x) * WAVES + wave;
     const int ntasks = (p.n_windows + PER - 1) / PER;
     if (task >= ntasks) return;                 // whole wave idle
     const int slot = lane / W, t = lane % W;

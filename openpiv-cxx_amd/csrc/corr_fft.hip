@@ -363,19 +363,6 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
 template <int W, bool BATCH>
 constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (BATCH ? 4 : 5) : W == 32 ? 3 : 2; }
 
-// Workgroups are dealt to the XCDs round-robin (block b to XCD b % 8), and
-// each XCD has its own L2.  Renumbering the blocks so that those of one XCD
-// take one contiguous run of the grid keeps overlapping windows (which share
-// half their pixels) in the same L2.  A bijection of [0, g) for any g, so it
-// changes only which XCD computes a window.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_block(int b, int g)
-{
-    const int x = b % kXcds, k = b / kXcds;
-    const int q = g / kXcds, r = g % kXcds;      // XCDs x < r take q + 1 blocks, the others q
-    return x * q + min(x, r) + k;
-}
-
 // BATCH reads each window's pair from the device table (per-lane frame
 // pointers) and loops over wave tasks, so the launch may be persistent; a
 // single-pair launch has uniform geometry and one task per wave.
@@ -390,7 +377,9 @@ void corr_fft_kernel(CorrParams p)
     const int wave = threadIdx.x >> 6;
     const int ntasks = (p.n_windows + ROUNDS * WPW - 1) / (ROUNDS * WPW);
     if constexpr (BATCH) {
-        for (int task = blockIdx.x * WAVES + wave; task < ntasks; task += gridDim.x * WAVES)
+        // each XCD strides through its own contiguous share of the tasks
+        const XcdShare sh = xcd_share((int)blockIdx.x, (int)gridDim.x, ntasks);
+        for (int task = sh.lo + sh.rank * WAVES + wave; task < sh.hi; task += sh.blocks * WAVES)
             corr_task<W, DT, PLANES, ROUNDS, BATCH>(p, task, lds_all[wave][lane / W], lane);
     } else {
         const int task = xcd_block((int)blockIdx.x, (int)gridDim.x) * WAVES + wave;
