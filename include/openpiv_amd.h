@@ -217,6 +217,12 @@ int opiv_average_subtract(opiv_ctx* ctx, const void* const* frames, size_t n, ui
 int opiv_vjp_pair(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,
                   const double* cot, float* grad_a, float* grad_b);
 
+/* Same, device pointers only and fwd required, enqueued on the context
+ * stream without synchronising (the caller synchronises), as
+ * opiv_run_pair_async is to opiv_run_pair. */
+int opiv_vjp_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, const opiv_window_result* d_fwd,
+                        const double* d_cot, float* d_grad_a, float* d_grad_b);
+
 /* var[2i], var[2i+1] = sigma^2 * |d(u_i, v_i)/d(both frames)|^2: the
  * variance of (u_i, v_i) under i.i.d. pixel noise sigma (linear propagation) */
 int opiv_variance_pair(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,

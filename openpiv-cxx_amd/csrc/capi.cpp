@@ -354,6 +354,26 @@ static int stage_in(opiv_ctx* ctx, DevBuf& buf, const void* src, size_t bytes, c
 }
 
 // frames, forward results and geometry shared by the two adjoint entry points
+// the adjoint kernels' view of a plan over device frames and forward results
+static opiv::AdjParams adjoint_params(const opiv_plan* plan, const void* da, const void* db,
+                                      const opiv_window_result* dfwd)
+{
+    opiv::AdjParams p{};
+    p.a = da;
+    p.b = db;
+    p.pitch = plan->desc.pitch;
+    p.x_start = plan->x_start;
+    p.y_start = plan->y_start;
+    p.step_x = (int32_t)plan->off_x;
+    p.step_y = (int32_t)plan->off_y;
+    p.nx = (int32_t)plan->nx;
+    p.ny = (int32_t)plan->ny;
+    p.n_windows = (int32_t)((size_t)plan->nx * plan->ny);
+    p.img_h = (int32_t)plan->desc.img_h;
+    p.fwd = dfwd;
+    return p;
+}
+
 static int adjoint_setup(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,
                          opiv::AdjParams& p)
 {
@@ -376,20 +396,35 @@ static int adjoint_setup(opiv_plan* plan, const void* a, const void* b, const op
         HIP_TRY(launch_plan(plan, cp, ctx->stream));
         dfwd = ctx->stage_fwd.p;
     }
-    p = opiv::AdjParams{};
-    p.a = da;
-    p.b = db;
-    p.pitch = plan->desc.pitch;
-    p.x_start = plan->x_start;
-    p.y_start = plan->y_start;
-    p.step_x = (int32_t)plan->off_x;
-    p.step_y = (int32_t)plan->off_y;
-    p.nx = (int32_t)plan->nx;
-    p.ny = (int32_t)plan->ny;
-    p.n_windows = (int32_t)count;
-    p.img_h = (int32_t)plan->desc.img_h;
-    p.fwd = (const opiv_window_result*)dfwd;
+    p = adjoint_params(plan, da, db, (const opiv_window_result*)dfwd);
     return OPIV_OK;
+}
+
+// the VJP's second half: tap weights and gradients into device buffers, on
+// the context stream
+static int launch_vjp(opiv_plan* plan, opiv::AdjParams& p, const double* d_cot, float* d_ga, float* d_gb)
+{
+    opiv_ctx* ctx = plan->ctx;
+    HIP_TRY(ctx->adj_coef.ensure((size_t)p.n_windows * sizeof(opiv::AdjCoef)));
+    p.coef = (opiv::AdjCoef*)ctx->adj_coef.p;
+    p.cot = d_cot;
+    p.grad_a = d_ga;
+    p.grad_b = d_gb;
+    p.gpitch = plan->desc.pitch;
+    HIP_TRY(opiv::launch_adjoint(p, (int)plan->desc.win, plan->desc.dtype, false, ctx->stream));
+    return OPIV_OK;
+}
+
+int opiv_vjp_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, const opiv_window_result* d_fwd,
+                        const double* d_cot, float* d_grad_a, float* d_grad_b)
+{
+    if (!plan || !d_a || !d_b || !d_fwd || !d_cot || !d_grad_a || !d_grad_b)
+        return fail(OPIV_E_INVALID, "opiv_vjp_pair_async: null argument");
+    if (plan->desc.variant == OPIV_REAL_COMPAT)
+        return fail(OPIV_E_UNSUPPORTED, "the adjoint supports the circular correlation only");
+    HIP_TRY(hipSetDevice(plan->ctx->device));
+    opiv::AdjParams p = adjoint_params(plan, d_a, d_b, d_fwd);
+    return launch_vjp(plan, p, d_cot, d_grad_a, d_grad_b);
 }
 
 int opiv_vjp_pair(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,
@@ -416,13 +451,8 @@ int opiv_vjp_pair(opiv_plan* plan, const void* a, const void* b, const opiv_wind
         HIP_TRY(ctx->stage_gb.ensure(gbytes));
         dgb = (float*)ctx->stage_gb.p;
     }
-    HIP_TRY(ctx->adj_coef.ensure(count * sizeof(opiv::AdjCoef)));
-    p.coef = (opiv::AdjCoef*)ctx->adj_coef.p;
-    p.cot = (const double*)dcot;
-    p.grad_a = dga;
-    p.grad_b = dgb;
-    p.gpitch = plan->desc.pitch;
-    HIP_TRY(opiv::launch_adjoint(p, (int)plan->desc.win, plan->desc.dtype, false, ctx->stream));
+    rc = launch_vjp(plan, p, (const double*)dcot, dga, dgb);
+    if (rc != OPIV_OK) return rc;
     if (!ga_dev) HIP_TRY(hipMemcpyAsync(grad_a, dga, gbytes, hipMemcpyDeviceToHost, ctx->stream));
     if (!gb_dev) HIP_TRY(hipMemcpyAsync(grad_b, dgb, gbytes, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

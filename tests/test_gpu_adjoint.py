@@ -136,3 +136,26 @@ def test_adjoint_rejects_real_variant(gpu_ctx):
     with pytest.raises(oa.OpivError) as e:
         plan.variance(z, z, 1.0)
     assert e.value.code == 6
+
+
+def test_vjp_async_matches_sync(gpu_ctx):
+    """opiv_vjp_pair_async (device buffers, no synchronisation) writes the
+    same gradients, bit for bit, as opiv_vjp_pair; missing forward results
+    are an error for it."""
+    a, b = frames(512, 512, seed=13)
+    plan = oa.Plan(gpu_ctx, 512, 512, 32, dtype=np.float32)
+    cot = np.random.default_rng(3).normal(size=(plan.count, 2))
+    fwd = plan.run(a, b)
+    ga, gb = plan.vjp(a, b, cot, fwd=fwd)
+    lib = oa.load()
+    da, db = oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)
+    dfwd, dcot = oa.DeviceBuffer.from_array(fwd), oa.DeviceBuffer.from_array(cot)
+    dga, dgb = oa.DeviceBuffer(ga.nbytes), oa.DeviceBuffer(gb.nbytes)
+    for _ in range(2):                               # second call reuses the context's tap-weight buffer
+        oa._check(lib.opiv_vjp_pair_async(plan.handle, da.ptr, db.ptr, dfwd.ptr, dcot.ptr, dga.ptr, dgb.ptr))
+        gpu_ctx.synchronize()
+        assert dga.download(np.empty_like(ga)).tobytes() == ga.tobytes()
+        assert dgb.download(np.empty_like(gb)).tobytes() == gb.tobytes()
+    with pytest.raises(oa.OpivError) as e:
+        oa._check(lib.opiv_vjp_pair_async(plan.handle, da.ptr, db.ptr, None, dcot.ptr, dga.ptr, dgb.ptr))
+    assert e.value.code == 1
