@@ -248,10 +248,9 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
 
     // ---- forward transforms; with two rounds the spectra pack as
     // Q = P0 + i 2^k P1, whose inverse is c0 + i 2^k c1 (both planes are
-    // real).  PREFETCH would load the second window's columns before the
-    // first window is transformed; it costs the registers of a third wave
-    // per SIMD at 32 x 32, and is off. ----
-    constexpr bool PREFETCH = false;
+    // real).  The second window's columns are loaded after the first
+    // window's spectrum: loading them earlier would cost the registers of a
+    // third wave per SIMD at 32 x 32. ----
     float2 q[W];
     float inv_scale = 1.f;
     const bool compat = p.real_compat != 0;
@@ -259,7 +258,6 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     float2 z0[W], z1[W];
     const TaskFrames frames = task_frames<DT>(geo[0]);
     load_column<W, DT>(frames, geo[0], t, z0);
-    if constexpr (PREFETCH) load_column<W, DT>(frames, geo[ROUNDS - 1], t, z1);
     const float zmax0 = forward_spectrum<W>(z0, t, lds);
     float2 h0[H2 + 1];
     half_product<W>(z0, t, lds, h0);
@@ -267,7 +265,7 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     // kx = 1..W/2-1: Q(-t, -kx) = conj P0(t, kx) + i 2^k conj P1(t, kx)
     float2 r[H2 + 1];
     if constexpr (ROUNDS == 2) {
-        if constexpr (!PREFETCH) load_column<W, DT>(frames, geo[ROUNDS - 1], t, z1);
+        load_column<W, DT>(frames, geo[ROUNDS - 1], t, z1);
         const float zmax1 = forward_spectrum<W>(z1, t, lds);
         const float scale = balance_scale(zmax0, zmax1);
         inv_scale = 1.f / scale;
