@@ -51,13 +51,15 @@ typedef enum opiv_variant {
      * unpack (fft.h:166-183) that leaves spectral row 0, row H/2 and
      * column 0 zero */
     OPIV_REAL_COMPAT = 1,
-    /* the exact circular correlation computed directly (no FFT) as an
-     * im2col GEMM on BF16 MFMA with split operands and FP32 accumulation:
-     * u8 exact, u16 exact products, f32/f64 three-term split; 16x16 windows */
+    /* the circular correlation computed directly (no FFT) as an im2col GEMM
+     * on MFMA: u8 frames exactly on int8 MFMA (pixels offset by 128), u16
+     * exact bf16 products, f32/f64 a three-term bf16 split, FP32
+     * accumulation; 16x16 windows */
     OPIV_DIRECT_BF16_SPLIT = 2,
     /* the faster of OPIV_CIRCULAR_FFT_FP32 and OPIV_DIRECT_BF16_SPLIT for
      * the plan's window size and pixel type as measured on MI355X (same
-     * semantics); at present the FFT path for every size and type */
+     * semantics): the direct path for 8-bit frames at 16x16, the FFT path
+     * otherwise */
     OPIV_AUTO = 3
 } opiv_variant;
 

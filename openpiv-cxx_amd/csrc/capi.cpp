@@ -125,14 +125,13 @@ int opiv_grid(uint32_t img_w, uint32_t img_h, uint32_t win_w, uint32_t win_h,
 // (105.6 us, and its correlation is exact) against the FFT kernel's 109.4 us;
 // for 16-bit and float frames the FFT kernel (110 / 109 us against 200 / 312
 // us direct).  Other window sizes have no direct kernel.
-// OPIV_AUTO: the faster path as measured on MI355X at 4096 x 4096.  The FFT
-// kernel wins at every window size and pixel type (16 x 16: 94 us against the
-// direct kernel's 101 us for u8, 200 us for u16, 310 us for f32).
+// OPIV_AUTO: the faster path as measured on MI355X at 4096 x 4096: the direct
+// kernel for 8-bit frames at 16 x 16 (exact int8 MFMA, 74 us against the FFT
+// kernel's 83 us), the FFT kernel otherwise (16 x 16 u16: 90 against 198 us,
+// f32: 89 against 309 us).
 static int32_t auto_variant(int win, int dtype)
 {
-    (void)win;
-    (void)dtype;
-    return OPIV_CIRCULAR_FFT_FP32;
+    return opiv::direct_supported(win) && dtype == OPIV_U8 ? OPIV_DIRECT_BF16_SPLIT : OPIV_CIRCULAR_FFT_FP32;
 }
 
 int opiv_plan_create(opiv_ctx* ctx, const opiv_plan_desc* desc, opiv_plan** out)

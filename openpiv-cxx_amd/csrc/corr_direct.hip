@@ -1,6 +1,6 @@
-// corr_direct.hip -- direct circular cross-correlation on BF16 MFMA with
-// split operands (windows of 16 x 16), fused with find_peaks and the
-// Gaussian fit.  Same semantics and outputs as the FFT kernel
+// corr_direct.hip -- direct circular cross-correlation on MFMA (int8 for
+// 8-bit frames, split bf16 operands otherwise; windows of 16 x 16), fused with
+// find_peaks and the Gaussian fit.  Same semantics and outputs as the FFT kernel
 // (corr_fft.hip): algos::FFT::cross_correlate (fft.h:188-206) +
 // swap_quadrants + find_peaks(.., 2, 1) + fit_simple_gaussian + the vector
 // of examples/process/main.cpp:220-234.
@@ -10,26 +10,29 @@
 //                                          Q[(y,x)][sx] = b[y][(x + sx) % W]
 // (M = N = 16, K = 256).  Both operands are read from LDS copies of the
 // window: P as aligned 16-byte row pieces of a (the row index carries the
-// shift), Q from doubled rows of b held twice, the second copy shifted by one
-// element, so that every 8-element run starting at any column is
+// shift), Q from doubled rows of b held in shifted copies (two for bf16
+// elements, four for bytes), so that every run starting at any column is
 // dword-aligned in one of the copies.
 //
-// Precision: pixels are split into bf16 terms, x = x0 + x1 (+ x2), and the
-// cross products are accumulated in FP32 by the MFMA:
-//   u8  : x0 only (exact; products and sums of 256 terms stay exact in FP32)
+// Precision: 8-bit frames run on int8 MFMA (pixels offset by 128, the exact
+// integer correlation; see stage8 / gemm8 below).  Other pixels are split
+// into bf16 terms, x = x0 + x1 (+ x2), and the cross products are
+// accumulated in FP32 by the MFMA:
 //   u16 : x0 + x1 exactly (8 + 8 bits), all four products
 //   f32 : three terms (f64 frames are narrowed to f32 first), the six
 //         products x_i y_j with i + j <= 2 (relative error ~2^-24, Henry,
 //         Tang & Heinecke, "Leveraging the bfloat16 AI datatype for
 //         higher-precision computations", ARITH 2019)
-// Each cross product (and, for u8, each K-chunk parity) has its own FP32
-// accumulator; the partial sums are added smallest product first.
+// Each cross product has its own FP32 accumulator; the partial sums are
+// added smallest product first.
 //
 // Mapping: a wave runs 8 windows through the MFMA one after another, then
 // finishes all 8 with the tail shared with the FFT kernel (one FP64 log per
-// lane for 8 windows).  MFMA 16x16x32 lane layout (lane l): A[i][k], B[k][j]
-// with i = j = l % 16 and k = 8 (l / 16) + 0..7; the result D[i][j] has
-// j = l % 16 and i = 4 (l / 16) + r in accumulator r.
+// lane for 8 windows).  MFMA lane layout (lane l): A[i][k], B[k][j] with
+// i = j = l % 16, lane group l / 16 holding one run of k (8 bf16 values of
+// the 16x16x32 form, 16 bytes of the 16x16x64 int8 form, the same k for the
+// same position in A and B); the result D[i][j] has j = l % 16 and
+// i = 4 (l / 16) + r in accumulator r.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -205,26 +208,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
         });
     };
 
-    // ---- C = P Q on MFMA: 8 k-chunks of 32, one accumulator per cross
-    // product x_i y_j (u16: all four; f32: i + j <= 2), and for u8 one per
-    // chunk parity, so the dependent MFMA chains are short ----
-    auto gemm = [&]() {
-        constexpr int NACC = NP == 1 ? 2 : (NP == 2 ? 4 : 6);
+    // ---- C = P Q on bf16 MFMA (u16, f32, f64 frames): 8 k-chunks of 32, one
+    // accumulator per cross product x_i y_j (u16: all four; f32: i + j <= 2),
+    // so the dependent MFMA chains are short.  Generic, so that it is
+    // compiled only for the pixel types that call it. ----
+    auto gemm = [&](auto...) {
+        static_assert(NP >= 2, "8-bit frames take the int8 path");
+        constexpr int NACC = NP == 2 ? 4 : 6;
         floatx4 part[NACC];
         sfor<NACC>([&](auto Q) { part[decltype(Q)::value] = floatx4{0.f, 0.f, 0.f, 0.f}; });
-        // For u8 (one term, LDS-read bound) the A operand of chunk c is that of
-        // chunk c - 1 moved two lanes along each 16-lane row (row i16 of P at
-        // rows y = 2c + g/2 reads a[(y - i16) mod 16]): DPP row_ror:2 instead
-        // of an LDS read.
-        constexpr bool AROT = NP == 1;
-        uint32_t Aw[NP][4];
-        if constexpr (AROT) {
-            const uint4 v = *reinterpret_cast<const uint4*>(&lds_a[0][((g >> 1) - i16) & (W - 1)][8 * (g & 1)]);
-            Aw[0][0] = v.x;
-            Aw[0][1] = v.y;
-            Aw[0][2] = v.z;
-            Aw[0][3] = v.w;
-        }
         sfor<8>([&](auto Cc) {
             constexpr int c = decltype(Cc)::value;
             const int ra = (2 * c + (g >> 1) - i16) & (W - 1);
@@ -234,24 +226,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
             bf16x8 A[NP], B[NP];
             sfor<NP>([&](auto K) {
                 constexpr int k = decltype(K)::value;
-                if constexpr (AROT) {
-                    if constexpr (c > 0) {
-                        sfor<4>([&](auto Mm) {
-                            constexpr int m = decltype(Mm)::value;
-                            Aw[k][m] = (uint32_t)__builtin_amdgcn_mov_dpp((int)Aw[k][m], 0x122, 0xf, 0xf, false);
-                        });
-                    }
-                    A[k] = __builtin_bit_cast(bf16x8, Aw[k]);
-                } else {
-                    A[k] = *reinterpret_cast<const bf16x8*>(&lds_a[k][ra][8 * (g & 1)]);
-                }
+                A[k] = *reinterpret_cast<const bf16x8*>(&lds_a[k][ra][8 * (g & 1)]);
                 const uint32_t* q = reinterpret_cast<const uint32_t*>(&lds_b[k][copy][rb * 2 * W + base]);
                 uint32_t w4[4] = {q[0], q[1], q[2], q[3]};
                 B[k] = __builtin_bit_cast(bf16x8, w4);
             });
-            if constexpr (NP == 1) {
-                part[c & 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[0], part[c & 1], 0, 0, 0);
-            } else if constexpr (NP == 2) {
+            if constexpr (NP == 2) {
                 part[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[1], part[0], 0, 0, 0);
                 part[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B[0], part[1], 0, 0, 0);
                 part[2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B[1], part[2], 0, 0, 0);
@@ -277,11 +257,90 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
         return acc;
     };
 
+    // ---- 8-bit frames: the exact integer correlation on
+    // v_mfma_i32_16x16x64_i8.  Pixels enter as v - 128 (the byte with its
+    // top bit flipped, a signed byte); for a circular correlation
+    //   sum_n (a' + 128)(b' + 128)[n + s] = C'[s] + 128 (Sa' + Sb') + 256 * 128^2,
+    // Sa', Sb' the window sums, and every term is an integer below 2^24, so
+    // the FP32 plane is exact.  K-chunk c (64 = 4 rows x 16 columns) gives
+    // lane group g window row y = 4c + g: the A operand of lane i16 is row
+    // (y - i16) mod 16 of a' (chunk c + 1's is chunk c's of lane i16 - 4: DPP
+    // row_ror:4), the B operand row y of b' rotated by i16, read from one of
+    // four byte-shifted copies of the doubled row so that it is dword-aligned.
+    // B rows are padded to 16 dwords and the copies offset by 260 dwords, so
+    // the 64 lanes of a B read hit 64 distinct banks. ----
+    constexpr int B8ROW = 16, B8COPY = 16 * B8ROW + 4;
+    uint32_t (&a8)[W][4] = *reinterpret_cast<uint32_t (*)[W][4]>(&lds[wave][0]);
+    uint32_t (&b8)[4][B8COPY] = *reinterpret_cast<uint32_t (*)[4][B8COPY]>(&lds[wave][W * 16]);
+    static_assert(W * 16 + 4 * B8COPY * 4 <= BYTES, "8-bit staging fits the wave's LDS");
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    // stage window i; returns Sa' + Sb' (every lane)
+    auto stage8 = [&](auto I) -> int {
+        constexpr int i = decltype(I)::value;
+        uint32_t pa = 0, pb = 0;
+        sfor<4>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            pa |= (uint32_t)ra_[i % DEPTH].v[j] << (8 * j);
+            pb |= (uint32_t)rb_[i % DEPTH].v[j] << (8 * j);
+        });
+        if constexpr (i + DEPTH < PER)
+            fetch(std::integral_constant<int, i + DEPTH>{}, ra_[i % DEPTH], rb_[i % DEPTH]);
+        pa ^= 0x80808080u;
+        pb ^= 0x80808080u;
+        // b'[srow][(scol + 4) % 16 ..] from the next lane of the quad
+        const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pb, 0x39, 0xf, 0xf, false);
+        a8[srow][lane & 3] = pa;
+        sfor<4>([&](auto Jc) {                  // copy j: byte m holds b'[srow][(m + j) % 16]
+            constexpr int j = decltype(Jc)::value;
+            const uint32_t w = j == 0 ? pb : __builtin_amdgcn_alignbyte(nb, pb, j);
+            b8[j][srow * B8ROW + (lane & 3)] = w;
+            b8[j][srow * B8ROW + (lane & 3) + 4] = w;
+        });
+        int sum = __builtin_amdgcn_sdot4((int)pa, 0x01010101, 0, false) +
+                  __builtin_amdgcn_sdot4((int)pb, 0x01010101, 0, false);
+        sfor<6>([&](auto K) {
+            constexpr int off = 1 << decltype(K)::value;
+            sum += group_partner<off>(sum);
+        });
+        return sum;
+    };
+    auto gemm8 = [&](int corr) {
+        v4i acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        const uint4 a0 = *reinterpret_cast<const uint4*>(&a8[(g - i16) & (W - 1)][0]);
+        uint32_t Aw[4] = {a0.x, a0.y, a0.z, a0.w};
+        const uint32_t* brow = &b8[i16 & 3][g * B8ROW + (i16 >> 2)];
+        sfor<4>([&](auto Cc) {
+            constexpr int c = decltype(Cc)::value;
+            if constexpr (c > 0) {
+                sfor<4>([&](auto Mm) {
+                    constexpr int m = decltype(Mm)::value;
+                    Aw[m] = (uint32_t)__builtin_amdgcn_mov_dpp((int)Aw[m], 0x124, 0xf, 0xf, false);
+                });
+            }
+            const uint32_t* q = brow + 4 * c * B8ROW;
+            const v4i A = {(int)Aw[0], (int)Aw[1], (int)Aw[2], (int)Aw[3]};
+            const v4i B = {(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+            acc[c & 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B, acc[c & 1], 0, 0, 0);
+        });
+        floatx4 out;
+        sfor<4>([&](auto R) {
+            constexpr int r = decltype(R)::value;
+            out[r] = (float)(acc[0][r] + acc[1][r] + corr);
+        });
+        return out;
+    };
+
     sfor<PER>([&](auto I) {
         constexpr int i = decltype(I)::value;
-        stage(I);
-        wave_sync();
-        accs[i] = gemm();
+        if constexpr (DT == 0) {
+            const int sums = stage8(I);
+            wave_sync();
+            accs[i] = gemm8(128 * sums + W * W * 128 * 128);
+        } else {
+            stage(I);
+            wave_sync();
+            accs[i] = gemm();
+        }
         wave_sync();                            // staging buffers free for the next window
     });
 

@@ -82,8 +82,8 @@ def test_direct_limit_search_and_auto(gpu_ctx):
     a, b = oa.synth_pair(640, 480, seed=9, params=(-2.2, 1.4), dtype=np.float32)
     plan = oa.Plan(gpu_ctx, 640, 480, 16, dtype=np.float32, variant=D, limit_search=True)
     compare(plan.run(a, b), orc.process_pair(a, b, 16, 8, 8, limit_search=True, threads=8), "direct-limit")
-    # AUTO: the FFT kernel (the faster path for every size and type)
-    for dtype, chosen in ((np.float32, oa.CIRCULAR_FFT_FP32), (np.uint8, oa.CIRCULAR_FFT_FP32)):
+    # AUTO: the direct path for 8-bit frames at 16 x 16, the FFT kernel otherwise
+    for dtype, chosen in ((np.float32, oa.CIRCULAR_FFT_FP32), (np.uint8, D)):
         fa, fb = (a, b) if dtype == np.float32 else (np.clip(a, 0, 255).astype(np.uint8),
                                                        np.clip(b, 0, 255).astype(np.uint8))
         assert oa.direct_chosen(16, dtype) == (chosen == D)
