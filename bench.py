@@ -35,6 +35,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "openpiv-cxx_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# MI355X published dense peaks (TFLOP/s or TOP/s): FP32 vector, bf16 and int8 MFMA
+FP32_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+I8_MFMA_PEAK_TOPS = 5000.0
 METRIC = "vectors/sec at 32² win 50% overlap, chosen vs plain GPU path at ≤1e-3 px"
 RESULT_BYTES = 72
 
@@ -248,6 +252,20 @@ def bench_background(steps: int, warmup: int, nframes: int = 16):
 BATCH_SIZES = [(512, 512), (1024, 1024), (2048, 2048), (4096, 4096), (1280, 800)]
 
 
+def fft_flops(win: int) -> float:
+    """Algorithmic flops of one window on the FFT path: 5 N log2 N per complex
+    N-point transform, halved for real input (two forward transforms, the
+    half-spectrum product, one inverse)."""
+    return 7.5 * win * win * np.log2(win * win) + 3 * win * win
+
+
+def batch_compute(per_pass, wins, ms):
+    flops = sum(n * fft_flops(w) for n, w in zip(per_pass, wins))
+    rate = flops / (ms * 1e-3) / 1e12
+    return {"bound": "fp32_valu", "achieved": round(rate, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(rate / FP32_PEAK_TFLOPS, 4), "flops_per_step": round(flops)}
+
+
 def bench_batch(steps: int, warmup: int):
     """24 frame pairs with seeded sizes from BATCH_SIZES, each with a seeded
     Lamb-Oseen vortex (core radius 40 px, peak 5 px), passes 64 -> 32 -> 16 at
@@ -295,7 +313,8 @@ def bench_batch(steps: int, warmup: int):
             "roofline": {"bound": "hbm", "achieved": round((3 * 2 * 4 * npx + batch.count * RESULT_BYTES)
                                                            / (ms * 1e-3) / 1e9, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "note": "algorithmic bytes: every pass reads both frames once"}}
+                         "note": "algorithmic bytes: every pass reads both frames once"},
+            "compute_roofline": batch_compute(per_pass, (64, 32, 16), ms)}
     print(json.dumps(line), flush=True)
 
 
@@ -411,26 +430,52 @@ def main():
                 "algorithmic_bytes_per_launch": alg_bytes,
                 "bytes_per_window": round(alg_bytes / n, 1)}
 
+    # compute side of the roofline.  FFT path: F(W) = 7.5 W^2 log2(W^2) + 3 W^2
+    # flops per window (5 N log2 N per complex N-point transform, halved for
+    # real input: two forward transforms, the half-spectrum product, one
+    # inverse) against the FP32 vector peak.  Direct path: the window's GEMM,
+    # 2 W^4 operations per product (one int8 product for 8-bit frames, 4 or 6
+    # split bf16 products otherwise) against the MFMA peak of its type.
+    if chosen == "fft":
+        per_win = fft_flops(win)
+        peak, unit, kind = FP32_PEAK_TFLOPS, "TFLOP/s", "fp32_valu"
+    elif np_dtype == np.uint8:
+        per_win = 2.0 * win ** 4
+        peak, unit, kind = I8_MFMA_PEAK_TOPS, "TOP/s", "mfma_i8"
+    else:
+        per_win = 2.0 * win ** 4 * (4 if np_dtype == np.uint16 else 6)
+        peak, unit, kind = BF16_MFMA_PEAK_TFLOPS, "TFLOP/s", "mfma_bf16"
+    rate = per_win * n / (ms_per_step * 1e-3) / 1e12
+    floor_us = per_win * n / (peak * 1e12) * 1e6
+    hbm_floor_us = alg_bytes / (HBM_PEAK_GBS * 1e9) * 1e6
+    compute = {"bound": kind, "achieved": round(rate, 2), "peak": peak, "unit": unit,
+               "frac": round(rate / peak, 4), "ops_per_window": round(per_win, 1),
+               "floor_us": round(floor_us, 2), "hbm_floor_us": round(hbm_floor_us, 2),
+               "binding_floor": kind if floor_us > hbm_floor_us else "hbm"}
+
     cpu = None
     if not args.no_cpu:
         threads = max(1, min(16, (os.cpu_count() or 2)) - 1)
         log(f"[bench] cpu baseline, {threads} threads")
         cpu = cpu_baseline(a, b, win, step, threads)
 
-    path_name = {"fft": "fft_fp32_fused", "direct": "direct_bf16_split_mfma"}[chosen]
+    direct_i8 = chosen == "direct" and np_dtype == np.uint8
+    path_name = {"fft": "fft_fp32_fused",
+                 "direct": "direct_i8_mfma" if direct_i8 else "direct_bf16_split_mfma"}[chosen]
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "vectors/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        # arithmetic of the correlation: FP32 FFT, or bf16 products accumulated in FP32
-        "dtype": "bf16" if chosen == "direct" else "f32", "data": "synthetic",
+        # arithmetic of the correlation: FP32 FFT, int8 products accumulated in
+        # int32 (8-bit frames on the direct path), or bf16 products accumulated in FP32
+        "dtype": "i8" if direct_i8 else ("bf16" if chosen == "direct" else "f32"), "data": "synthetic",
         "config": {"workload": f"4096x4096 {args.dtype} frame pair, {win}x{win} windows step {step} ({n} windows), "
                                f"correlation + find_peaks + Gaussian fit fused, chosen path {path_name}",
                    "windows": n, "window": win, "step": step, "path": path_name, "frame_dtype": args.dtype,
                    "frames_resident_in_hbm": True},
         "plain_gpu": {"path": "fft_fp32_fused", "ms_per_step": round(plain_ms, 5),
                       "value": round(n / (plain_ms * 1e-3), 1)},
-        "roofline": roofline, "cpu_baseline": cpu, "accuracy": acc,
+        "roofline": roofline, "compute_roofline": compute, "cpu_baseline": cpu, "accuracy": acc,
         "wall_s_timed_region": round(wall, 4),
     }
     print(json.dumps(line), flush=True)
