@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdint>
 
@@ -80,6 +81,40 @@ __device__ __forceinline__ float group_partner(float v)
     return __int_as_float(group_partner<OFF>(__float_as_int(v)));
 }
 
+// One step of a max (min) reduction with the partner of group_partner<OFF>.
+// Within a 16-lane row the exchange is the DPP operand of the max itself: in
+// compiled code a max of a DPP move first canonicalises the moved value, two
+// extra instructions of the slow (4+ cycle) class per step.  s_nop 1 gives
+// the two wait states a DPP read needs after a VALU write of its source.
+#define OPIV_DPP_STEP(OP, CTRL)                                                                        \
+    asm("s_nop 1\n\t" OP " %0, %1, %1 " CTRL " row_mask:0xf bank_mask:0xf" : "=v"(r) : "v"(v))
+template <int OFF>
+__device__ __forceinline__ float max_with_partner(float v)
+{
+    float r;
+    if constexpr (OFF == 1) OPIV_DPP_STEP("v_max_f32_dpp", "quad_perm:[1,0,3,2]");
+    else if constexpr (OFF == 2) OPIV_DPP_STEP("v_max_f32_dpp", "quad_perm:[2,3,0,1]");
+    else if constexpr (OFF == 4) OPIV_DPP_STEP("v_max_f32_dpp", "row_half_mirror");
+    else if constexpr (OFF == 8) OPIV_DPP_STEP("v_max_f32_dpp", "row_mirror");
+    else {
+        const float o = group_partner<OFF>(v);
+        asm("v_max_f32_e32 %0, %1, %2" : "=v"(r) : "v"(v), "v"(o));
+    }
+    return r;
+}
+template <int OFF>
+__device__ __forceinline__ int min_with_partner(int v)
+{
+    int r;
+    if constexpr (OFF == 1) OPIV_DPP_STEP("v_min_i32_dpp", "quad_perm:[1,0,3,2]");
+    else if constexpr (OFF == 2) OPIV_DPP_STEP("v_min_i32_dpp", "quad_perm:[2,3,0,1]");
+    else if constexpr (OFF == 4) OPIV_DPP_STEP("v_min_i32_dpp", "row_half_mirror");
+    else if constexpr (OFF == 8) OPIV_DPP_STEP("v_min_i32_dpp", "row_mirror");
+    else r = min(v, group_partner<OFF>(v));
+    return r;
+}
+#undef OPIV_DPP_STEP
+
 // value of lane + D of the same 16-lane row (row_ror by 16 - D); the caller
 // keeps lane % 16 + D below 16
 template <int D>
@@ -109,13 +144,6 @@ __device__ __forceinline__ float load_px(const void* base, size_t idx)
 }
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
-
-// ordering of find_peaks' sort (descending value); equal values keep scan
-// order, i.e. the smaller row-major index comes first
-__device__ __forceinline__ bool better(float va, int ia, float vb, int ib)
-{
-    return va > vb || (va == vb && ia < ib);
-}
 
 } // namespace
 
@@ -182,6 +210,20 @@ __device__ __forceinline__ WinGeom window_geom(const CorrParams& p, int local)
     return g;
 }
 
+// max / min over the lanes of each W-lane group, every lane receiving it
+template <int W>
+__device__ __forceinline__ float group_max(float v)
+{
+    sfor<ilog2(W)>([&](auto K) { v = max_with_partner<1 << decltype(K)::value>(v); });
+    return v;
+}
+template <int W>
+__device__ __forceinline__ int group_min(int v)
+{
+    sfor<ilog2(W)>([&](auto K) { v = min_with_partner<1 << decltype(K)::value>(v); });
+    return v;
+}
+
 // find_peaks(.., 2, 1) over one quadrant-swapped plane column held by lane
 // t: strict 4-neighbour maxima on rows [hlo, hhi] and columns [wlo, whi]
 // (image_utils.impl.h:48-64), reduced to the largest value (and its scan
@@ -208,28 +250,14 @@ __device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, i
         v1 = fmaxf(v1, mc);                       // = gt ? mc : v1 (a max keeps v1 provably canonical)
     });
     i1 = h1 * W + t;
-    // top-2 merge of disjoint lane sets, ascending partner distance
-    sfor<ilog2(W)>([&](auto K) {
-        constexpr int off = 1 << decltype(K)::value;
-        const float ov1 = group_partner<off>(v1);
-        const float ov2 = group_partner<off>(v2);
-        const int oi1 = group_partner<off>(i1);
-        v2 = fmaxf(fmaxf(v2, ov2), fminf(v1, ov1));
-        const bool mine = better(v1, i1, ov1, oi1);
-        v1 = mine ? v1 : ov1;
-        i1 = mine ? i1 : oi1;
-    });
-}
-
-// max over the lanes of each W-lane group
-template <int W>
-__device__ __forceinline__ float group_max(float v)
-{
-    sfor<ilog2(W)>([&](auto K) {
-        constexpr int off = 1 << decltype(K)::value;
-        v = fmaxf(v, group_partner<off>(v));
-    });
-    return v;
+    // top 2 of the group: the largest value; among the lanes holding it the
+    // smallest index (scan order; indices of different lanes differ); then
+    // the largest of that lane's second value and every other lane's first
+    // (a tie with another lane's first gives v2 == v1, as the sort does)
+    const float gv1 = group_max<W>(v1);
+    i1 = group_min<W>(v1 == gv1 ? i1 : INT_MAX);
+    v2 = group_max<W>(i1 == h1 * W + t ? v2 : v1);
+    v1 = gv1;
 }
 
 // Results of the ROUNDS windows of this lane group from their swapped planes

@@ -158,10 +158,9 @@ __device__ __forceinline__ void lds_exchange(LdsT<W> (*lds)[W + 1], Src&& src, W
 
 // Transform the window's column t (z, from load_column) in place: lane t
 // then holds spectral row ky = t of Z = FFT(a + i b), Z(t, kx) at
-// z[brev_n<W>(kx)].  Returns the window's max |Z| component, which bounds the
-// cross-power spectrum: |P| <= max|Z|^2.
+// z[brev_n<W>(kx)].
 template <int W>
-__device__ __forceinline__ float forward_spectrum(float2 (&z)[W], int t, LdsT<W> (*lds)[W + 1])
+__device__ __forceinline__ void forward_spectrum(float2 (&z)[W], int t, LdsT<W> (*lds)[W + 1])
 {
     // ---- FFT along y (column t), transpose through LDS ----
     fftreg::fft_dit<W, -1>(z);
@@ -173,12 +172,6 @@ __device__ __forceinline__ float forward_spectrum(float2 (&z)[W], int t, LdsT<W>
         [&](auto X, float2 v) { z[decltype(X)::value] = v; });
     // ---- FFT along x: lane t holds spectral row ky = t ----
     fftreg::fft_dit<W, -1>(z);
-    float zmax = 0.f;
-    sfor<W>([&](auto K) {
-        constexpr int k = decltype(K)::value;
-        zmax = fmaxf(zmax, fmaxf(fabsf(z[k].x), fabsf(z[k].y)));
-    });
-    return group_max<W>(zmax);
 }
 
 // Half of this lane's row of 4 P, P = B conj(A) the cross-power spectrum:
@@ -215,15 +208,28 @@ __device__ __forceinline__ void half_product(const float2 (&z)[W], int t, LdsT<W
         });
 }
 
-// The power of two 2^k that equalises the bounds max|Z|^2 of two packed
-// spectra, so that each plane's FP32 rounding stays relative to its own
-// magnitude; multiplying by it is exact.
-__device__ __forceinline__ float balance_scale(float zmax0, float zmax1)
+// The window's largest |component| of 4 P: the lane's half row covers its
+// row's other half through the mirror lane, so the group covers the plane.
+template <int W>
+__device__ __forceinline__ float spectrum_max(const float2 (&h)[W / 2 + 1])
+{
+    float m = 0.f;
+    sfor<W / 2 + 1>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        m = fmaxf(m, fmaxf(fabsf(h[k].x), fabsf(h[k].y)));
+    });
+    return group_max<W>(m);
+}
+
+// The power of two 2^k that equalises the largest components of two packed
+// cross-power spectra, so that each plane's FP32 rounding stays relative to
+// its own magnitude; multiplying by it is exact.
+__device__ __forceinline__ float balance_scale(float pmax0, float pmax1)
 {
     int e0, e1;
-    frexpf(zmax0, &e0);
-    frexpf(zmax1, &e1);
-    const int k = (zmax0 == 0.f || zmax1 == 0.f) ? 0 : max(-60, min(60, 2 * (e0 - e1)));
+    frexpf(pmax0, &e0);
+    frexpf(pmax1, &e1);
+    const int k = (pmax0 == 0.f || pmax1 == 0.f) ? 0 : max(-60, min(60, e0 - e1));
     return ldexpf(1.f, k);
 }
 
@@ -258,7 +264,7 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     float2 z0[W], z1[W];
     const TaskFrames frames = task_frames<DT>(geo[0]);
     load_column<W, DT>(frames, geo[0], t, z0);
-    const float zmax0 = forward_spectrum<W>(z0, t, lds);
+    forward_spectrum<W>(z0, t, lds);
     float2 h0[H2 + 1];
     half_product<W>(z0, t, lds, h0);
     // q[kx] for kx = 0..W/2, and r[kx] = the mirror lane's q[W - kx] for
@@ -266,11 +272,11 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     float2 r[H2 + 1];
     if constexpr (ROUNDS == 2) {
         load_column<W, DT>(frames, geo[ROUNDS - 1], t, z1);
-        const float zmax1 = forward_spectrum<W>(z1, t, lds);
-        const float scale = balance_scale(zmax0, zmax1);
-        inv_scale = 1.f / scale;
+        forward_spectrum<W>(z1, t, lds);
         float2 h1[H2 + 1];
         half_product<W>(z1, t, lds, h1);
+        const float scale = balance_scale(spectrum_max<W>(h0), spectrum_max<W>(h1));
+        inv_scale = 1.f / scale;
         sfor<H2 + 1>([&](auto K) {
             constexpr int k = decltype(K)::value;
             const float2 a = h0[k], c = h1[k];
