@@ -246,9 +246,12 @@ __device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, i
         const float mc = (ismax & inreg) ? c : -INFINITY;
         const bool gt = mc > v1;                  // rows arrive in scan order: ties keep the first
         v2 = __builtin_amdgcn_fmed3f(v2, mc, v1); // = max(v2, min(mc, v1)), as v2 <= v1
-        h1 = gt ? h : h1;
-        v1 = fmaxf(v1, mc);                       // = gt ? mc : v1 (a max keeps v1 provably canonical)
+        h1 = gt ? h : h1;                         // h is an inline constant of the select
+        v1 = gt ? mc : v1;                        // a select is cheaper than a max here
     });
+    // keeps the multiply out of the selects above (folded in, it turns every
+    // h into a non-inline constant held in a register)
+    asm("" : "+v"(h1));
     i1 = h1 * W + t;
     // top 2 of the group: the largest value; among the lanes holding it the
     // smallest index (scan order; indices of different lanes differ); then
