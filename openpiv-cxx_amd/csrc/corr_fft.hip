@@ -106,9 +106,9 @@ __device__ __forceinline__ void load_column(const TaskFrames& f, const WinGeom& 
 
 // From 16 x 16 up the lane group's LDS slice holds floats, and every
 // complex exchange goes through it in two passes (real parts, then imaginary
-// parts): half the LDS, so that five waves (16 x 16), three (32 x 32) or two
-// (64 x 64) fit per SIMD.  The kernel is bound by latency (LDS round trips,
-// loads), which more resident waves hide.
+// parts): half the LDS, so that six waves (16 x 16), four (32 x 32) or two
+// (64 x 64) fit per SIMD.  The kernel is bound by VALU issue; the resident
+// waves fill each other's waits on LDS round trips and loads.
 template <int W>
 constexpr bool split_lds() { return W >= 16; }
 template <int W>
@@ -361,9 +361,11 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     });
 }
 
-// occupancy target per window size: the most waves whose VGPR budget keeps
-// the register-resident lines without spilling; batch kernels carry per-lane
-// frame pointers and get a little more room at W = 16
+// occupancy floor per window size: the most waves whose VGPR budget keeps
+// the register-resident lines of every instance (plane output, f64 frames,
+// batch with per-lane frame pointers and a task loop) without spilling.  The
+// hot instances need fewer registers and run more waves: 74 VGPRs (6 waves)
+// at 16 x 16, 124 (4 waves) at 32 x 32.
 template <int W, bool BATCH>
 constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (BATCH ? 4 : 5) : W == 32 ? 3 : 2; }
 
