@@ -183,20 +183,27 @@ __device__ __forceinline__ WinGeom window_geom(const CorrParams& p, int local)
         g.img_h = p.img_h;
         g.out = l;
     } else {
-        const gptr_t<BatchPair> pairs = (gptr_t<BatchPair>)p.pairs;
+        // The table is written before the launch and only read here: the
+        // constant address space lets uniform reads of it be scalar loads.
+        const __attribute__((address_space(4))) BatchPair* pairs =
+            (const __attribute__((address_space(4))) BatchPair*)p.pairs;
         const int l = g.valid ? local : p.n_windows - 1;
+        // The windows of a wave task lie in one pair (pair ranges are
+        // task-aligned): the search runs on the first lane's index, so that
+        // the pair record is wave-uniform and stays in scalar registers.
+        const int l0 = __builtin_amdgcn_readfirstlane(l);
         int lo = 0, hi = p.npairs;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (pairs[mid].first <= l) lo = mid;
+            if (pairs[mid].first <= l0) lo = mid;
             else hi = mid;
         }
         const auto& bp = pairs[lo];
         const int n = bp.nx * bp.ny;
         int gid = l - bp.first;
-        if (gid >= n) {
+        if (gid < 0 || gid >= n) {               // (gid < 0 only guards the addressing)
             g.valid = false;
-            gid = n - 1;
+            gid = gid < 0 ? 0 : n - 1;
         }
         const int ix = gid % bp.nx, iy = gid / bp.nx;
         g.a = bp.a;

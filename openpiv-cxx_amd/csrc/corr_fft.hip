@@ -366,23 +366,28 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
 // batch with per-lane frame pointers and a task loop) without spilling.  The
 // hot instances need fewer registers and run more waves: 74 VGPRs (6 waves)
 // at 16 x 16, 124 (4 waves) at 32 x 32.
-template <int W, bool BATCH>
-constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (BATCH ? 4 : 5) : W == 32 ? 3 : 2; }
+template <int W, bool LOOP>
+constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (LOOP ? 4 : 5) : W == 32 ? 3 : 2; }
 
-// BATCH reads each window's pair from the device table (per-lane frame
-// pointers) and loops over wave tasks, so the launch may be persistent; a
-// single-pair launch has uniform geometry and one task per wave.
-template <int W, int DT, bool PLANES, int ROUNDS, int NT, bool BATCH>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves_per_simd<W, BATCH>())))
+// Launch modes: one pair, one task per wave; a batch (each window's pair
+// from the device table) with one task per wave; a batch whose waves loop
+// over tasks, so that the launch may be persistent.  The loop keeps its
+// invariants live across the whole task and costs registers: 16 x 16 and
+// 32 x 32 run one wave fewer per SIMD with it.
+enum LaunchMode { kOnePair = 0, kBatch = 1, kBatchLoop = 2 };
+
+template <int W, int DT, bool PLANES, int ROUNDS, int NT, int MODE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves_per_simd<W, MODE == kBatchLoop>())))
 void corr_fft_kernel(CorrParams p)
 {
     constexpr int WPW = 64 / W;
     constexpr int WAVES = NT / 64;
+    constexpr bool BATCH = MODE != kOnePair;
     __shared__ LdsT<W> lds_all[WAVES][WPW][W][W + 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int ntasks = (p.n_windows + ROUNDS * WPW - 1) / (ROUNDS * WPW);
-    if constexpr (BATCH) {
+    if constexpr (MODE == kBatchLoop) {
         // each XCD strides through its own contiguous share of the tasks
         const XcdShare sh = xcd_share((int)blockIdx.x, (int)gridDim.x, ntasks);
         for (int task = sh.lo + sh.rank * WAVES + wave; task < sh.hi; task += sh.blocks * WAVES)
@@ -405,12 +410,16 @@ static hipError_t launch_wn(const CorrParams& p, bool planes, int blocks, hipStr
     const int full = (ntasks + tasks_per_block - 1) / tasks_per_block;
     const int grid = blocks > 0 && blocks < full ? blocks : full;
     if constexpr (BATCH) {
-        hipLaunchKernelGGL((corr_fft_kernel<W, DT, false, ROUNDS, NT, true>), dim3(grid), dim3(NT), 0, stream, p);
+        if (grid == full)    // one task per wave: no loop
+            hipLaunchKernelGGL((corr_fft_kernel<W, DT, false, ROUNDS, NT, kBatch>), dim3(grid), dim3(NT), 0, stream, p);
+        else
+            hipLaunchKernelGGL((corr_fft_kernel<W, DT, false, ROUNDS, NT, kBatchLoop>), dim3(grid), dim3(NT), 0,
+                               stream, p);
     } else {
         if (planes)
-            hipLaunchKernelGGL((corr_fft_kernel<W, DT, true, ROUNDS, NT, false>), dim3(grid), dim3(NT), 0, stream, p);
+            hipLaunchKernelGGL((corr_fft_kernel<W, DT, true, ROUNDS, NT, kOnePair>), dim3(grid), dim3(NT), 0, stream, p);
         else
-            hipLaunchKernelGGL((corr_fft_kernel<W, DT, false, ROUNDS, NT, false>), dim3(grid), dim3(NT), 0, stream, p);
+            hipLaunchKernelGGL((corr_fft_kernel<W, DT, false, ROUNDS, NT, kOnePair>), dim3(grid), dim3(NT), 0, stream, p);
     }
     return hipGetLastError();
 }
