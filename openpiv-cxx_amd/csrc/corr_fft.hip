@@ -59,6 +59,27 @@ struct TaskFrames {
 template <int DT>
 constexpr uint32_t px_bytes() { return DT == 0 ? 1 : DT == 1 ? 2 : DT == 2 ? 4 : 8; }
 
+// Word 3 of a buffer resource.  f32/f64 frames use untyped loads (the data
+// format field only has to be valid: 32 bits).  8- and 16-bit frames use
+// typed loads of one component (destination X = component X) in the
+// unsigned-scaled number format, which return the pixel as a float: the
+// conversion happens in the load path instead of a VALU conversion (a
+// slow-class instruction per pixel).
+template <int DT>
+constexpr int rsrc_word3()
+{
+    constexpr int kDstSelX = 4, kNumFormatUscaled = 2 << 12;
+    constexpr int kDataFormat8 = 1 << 15, kDataFormat16 = 2 << 15, kDataFormat32 = 4 << 15;
+    return DT == 0 ? kDstSelX | kNumFormatUscaled | kDataFormat8
+         : DT == 1 ? kDstSelX | kNumFormatUscaled | kDataFormat16
+                   : kDataFormat32;
+}
+
+// typed buffer load of one component (the LLVM intrinsic; clang has no
+// builtin for it)
+extern "C" __device__ float opiv_buffer_load_format_f32(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset,
+                                                         int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.format.f32");
+
 __device__ __forceinline__ uint64_t first_lane(uint64_t v)
 {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
@@ -71,14 +92,13 @@ template <int DT>
 __device__ __forceinline__ TaskFrames task_frames(const WinGeom& g0)
 {
     constexpr uint32_t ES = px_bytes<DT>();
-    constexpr int kRawWord3 = 0x00020000;       // 32-bit data format, the untyped-access setting
     TaskFrames f;
     f.base = first_lane((uint64_t)g0.y0 * g0.pitch + (uint64_t)g0.x0);
     const uint64_t pitch = first_lane(g0.pitch);
     const uint64_t a = first_lane((uint64_t)(uintptr_t)g0.a) + f.base * ES;
     const uint64_t b = first_lane((uint64_t)(uintptr_t)g0.b) + f.base * ES;
-    f.a = __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)a, 0, -1, kRawWord3);
-    f.b = __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)b, 0, -1, kRawWord3);
+    f.a = __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)a, 0, -1, rsrc_word3<DT>());
+    f.b = __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)b, 0, -1, rsrc_word3<DT>());
     f.pitch_bytes = (uint32_t)(pitch * ES);
     return f;
 }
@@ -86,8 +106,7 @@ __device__ __forceinline__ TaskFrames task_frames(const WinGeom& g0)
 template <int DT>
 __device__ __forceinline__ float load_buf(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
 {
-    if constexpr (DT == 0) return (float)__builtin_amdgcn_raw_buffer_load_b8(r, off, soff, 0);
-    else if constexpr (DT == 1) return (float)__builtin_amdgcn_raw_buffer_load_b16(r, off, soff, 0);
+    if constexpr (DT <= 1) return opiv_buffer_load_format_f32(r, (int)off, (int)soff, 0);
     else if constexpr (DT == 2) return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
     else return (float)__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
 }
