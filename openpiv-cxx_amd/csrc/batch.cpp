@@ -132,7 +132,7 @@ int fill_table(opiv_batch* b, const opiv_pair_desc* pairs, size_t npairs)
         int rc = opiv_grid_offsets(win, win, d.overlap[ps], &ox, &oy);
         if (rc) return rc;
         int64_t first = 0;
-        const int64_t align = opiv::batch_align((int)win);
+        const int64_t align = opiv::batch_align((int)win, (int)d.dtype);
         const size_t pass_base = total;
         for (size_t i = 0; i < npairs; ++i) {
             const auto& pr = pairs[i];

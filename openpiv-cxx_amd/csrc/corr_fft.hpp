@@ -29,8 +29,10 @@ struct BatchPair {
     int32_t pad_;
 };
 
-// windows per wave task of the kernel for window size win
-constexpr int batch_align(int win) { return (win <= 32 ? 2 : 1) * (64 / win); }
+// windows per wave task of the kernel for window size win and pixel type
+// dtype (0 u8, 1 u16, 2 f32, 3 f64): two rounds of 64 / win lane groups, one
+// round for f64 frames at 64 x 64
+constexpr int batch_align(int win, int dtype) { return (win == 64 && dtype == 3 ? 1 : 2) * (64 / win); }
 
 // Largest frame row pitch in bytes.  The FFT kernel addresses a wave task's
 // pixels as 32-bit byte offsets from the task's first window: a task spans
