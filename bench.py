@@ -424,7 +424,10 @@ def main():
     frame_bytes = 2 * w * w * np.dtype(np_dtype).itemsize
     alg_bytes = frame_bytes + n * RESULT_BYTES
     achieved = alg_bytes / (ms_per_step * 1e-3) / 1e9
+    # measured traffic exists for the default grid step (half the window) only
     key = f"{'direct_bf16' if chosen == 'direct' else 'fft_fp32'}_w{win}_4096_{args.dtype}"
+    if step != win // 2:
+        key += f"_s{step}"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(key),
                 "algorithmic_bytes_per_launch": alg_bytes,

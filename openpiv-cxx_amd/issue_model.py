@@ -5,7 +5,7 @@ and the kernel time that sum implies when every SIMD issues without a gap.
     hipcc --offload-arch=gfx950 -O3 -std=c++17 -I../include -Icsrc \
           -Xclang -target-feature -Xclang -packed-fp32-ops \
           --cuda-device-only -S csrc/corr_fft.hip -o corr_fft.s
-    python3 issue_model.py corr_fft.s probe.txt corr_fft_kernelILi32ELi2ELb0ELi2ELi256ELb0E --waves 16260
+    python3 issue_model.py corr_fft.s probe.txt corr_fft_kernelILi32ELi2ELb0ELi2ELi256ELi0E --waves 16260
 
 `probe.txt` is the output of lib/opiv_issue_probe on the GPU.  The kernels
 are straight-line code (every loop over rows, windows and FFT stages is
