@@ -428,6 +428,7 @@ template <int W, int DT, int NT, bool BATCH>
 static hipError_t launch_wn(const CorrParams& p, bool planes, int blocks, hipStream_t stream)
 {
     constexpr int ROUNDS = rounds_for<W, DT>();
+    static_assert(batch_align(W, DT) == ROUNDS * (64 / W), "batch pair ranges align to the wave task");
     constexpr int tasks_per_block = NT / 64;
     const int ntasks = (p.n_windows + ROUNDS * (64 / W) - 1) / (ROUNDS * (64 / W));
     const int full = (ntasks + tasks_per_block - 1) / tasks_per_block;
