@@ -130,8 +130,9 @@ int opiv_plan_grid(const opiv_plan* plan, int32_t* xy);
 /* The whole processing loop of examples/process/main.cpp:241-302 for one
  * frame pair: every window's correlation, find_peaks(.., 2, 1),
  * fit_simple_gaussian and vector, in grid order.  a, b, out may be host or
- * device pointers; n must equal the plan's window count.  Returns when the
- * results are in `out`. */
+ * device pointers; n must equal the plan's window count.  Device frames (here
+ * and in every entry point below) must be aligned to their pixel size
+ * (OPIV_E_INVALID otherwise).  Returns when the results are in `out`. */
 int opiv_run_pair(opiv_plan* plan, const void* a, const void* b,
                   opiv_window_result* out, size_t n);
 

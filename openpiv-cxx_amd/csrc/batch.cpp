@@ -137,6 +137,9 @@ int fill_table(opiv_batch* b, const opiv_pair_desc* pairs, size_t npairs)
         for (size_t i = 0; i < npairs; ++i) {
             const auto& pr = pairs[i];
             if (!pr.a || !pr.b) return fail(OPIV_E_INVALID, "pair %zu: null frame pointer", i);
+            if (((uintptr_t)pr.a | (uintptr_t)pr.b) % dtype_size(d.dtype) != 0)
+                return fail(OPIV_E_INVALID, "pair %zu: frame pointers must be aligned to their %zu-byte pixels", i,
+                            dtype_size(d.dtype));
             const size_t pitch = pr.pitch ? pr.pitch : pr.img_w;
             if (pitch < pr.img_w) return fail(OPIV_E_INVALID, "pair %zu: pitch smaller than width", i);
             if (pitch * dtype_size(d.dtype) > opiv::kMaxPitchBytes)

@@ -203,6 +203,16 @@ int opiv_plan_grid(const opiv_plan* plan, int32_t* xy)
                      plan->off_x, plan->off_y, &nx, &ny, xy);
 }
 
+// Device frames must be aligned to their pixel size: 8- and 16-bit pixels
+// are read by typed buffer loads, which take the address of a whole element.
+static int check_frames(const opiv_plan* plan, const void* da, const void* db, const char* who)
+{
+    const size_t es = dtype_size(plan->desc.dtype);
+    if (((uintptr_t)da | (uintptr_t)db) % es != 0)
+        return fail(OPIV_E_INVALID, "%s: frame pointers must be aligned to their %zu-byte pixels", who, es);
+    return OPIV_OK;
+}
+
 static opiv::CorrParams plan_params(const opiv_plan* plan, const void* a, const void* b, opiv_window_result* out)
 {
     opiv::CorrParams p{};
@@ -238,6 +248,7 @@ int opiv_run_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, opiv_
     if (!plan || !d_a || !d_b || !d_out) return fail(OPIV_E_INVALID, "opiv_run_pair_async: null argument");
     const size_t count = (size_t)plan->nx * plan->ny;
     if (n != count) return fail(OPIV_E_SIZE, "result count %zu differs from the grid's %zu windows", n, count);
+    if (int rc = check_frames(plan, d_a, d_b, "opiv_run_pair_async")) return rc;
     HIP_TRY(hipSetDevice(plan->ctx->device));
     opiv::CorrParams p = plan_params(plan, d_a, d_b, d_out);
     HIP_TRY(launch_plan(plan, p, plan->ctx->stream));
@@ -264,6 +275,7 @@ int opiv_run_pair(opiv_plan* plan, const void* a, const void* b, opiv_window_res
         HIP_TRY(hipMemcpyAsync(ctx->stage_b.p, b, frame_bytes, hipMemcpyHostToDevice, ctx->stream));
         db = ctx->stage_b.p;
     }
+    if (int rc = check_frames(plan, da, db, "opiv_run_pair")) return rc;
     const bool out_dev = is_device_ptr(out);
     opiv_window_result* dout = out;
     if (!out_dev) {
@@ -385,6 +397,7 @@ static int adjoint_setup(opiv_plan* plan, const void* a, const void* b, const op
     const void *da, *db, *dfwd;
     int rc = stage_in(ctx, ctx->stage_a, a, frame_bytes, &da);
     if (rc == OPIV_OK) rc = stage_in(ctx, ctx->stage_b, b, frame_bytes, &db);
+    if (rc == OPIV_OK) rc = check_frames(plan, da, db, "adjoint");
     if (rc != OPIV_OK) return rc;
     if (fwd) {
         rc = stage_in(ctx, ctx->stage_fwd, fwd, count * sizeof(opiv_window_result), &dfwd);
@@ -421,6 +434,7 @@ int opiv_vjp_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, const
         return fail(OPIV_E_INVALID, "opiv_vjp_pair_async: null argument");
     if (plan->desc.variant == OPIV_REAL_COMPAT)
         return fail(OPIV_E_UNSUPPORTED, "the adjoint supports the circular correlation only");
+    if (int rc = check_frames(plan, d_a, d_b, "opiv_vjp_pair_async")) return rc;
     HIP_TRY(hipSetDevice(plan->ctx->device));
     opiv::AdjParams p = adjoint_params(plan, d_a, d_b, d_fwd);
     return launch_vjp(plan, p, d_cot, d_grad_a, d_grad_b);

@@ -296,3 +296,22 @@ def test_transform_matches_oracle(gpu_ctx, win):
     # real input: the transform of the real part is Hermitian
     r = gpu_ctx.transform(z[0].real, oa.FORWARD)
     assert np.abs(r - np.conj(r[(-np.arange(win)) % win][:, (-np.arange(win)) % win])).max() <= 1e-5 * np.abs(r).max()
+
+
+def test_misaligned_device_frames_rejected(gpu_ctx):
+    """Device frames must be aligned to their pixel size (typed 16-bit loads
+    take whole elements): an odd u16 frame address is an OPIV_E_INVALID
+    error, for the single pair and the asynchronous call alike."""
+    a, b = oa.synth_pair(128, 128, seed=3, params=(1.0, 0.5), dtype=np.uint16)
+    plan = oa.Plan(gpu_ctx, 128, 128, 32, dtype=np.uint16)
+    buf = oa.DeviceBuffer(a.nbytes + 2)
+    db = oa.DeviceBuffer.from_array(b)
+    out = oa.DeviceBuffer(plan.count * oa.RESULT_DTYPE.itemsize)
+    lib = oa.load()
+    for fn in (lib.opiv_run_pair_async, lib.opiv_run_pair):
+        with pytest.raises(oa.OpivError) as e:
+            oa._check(fn(plan.handle, buf.ptr + 1, db.ptr, out.ptr, plan.count))
+        assert e.value.code == 1
+    # the aligned address of the same buffer runs
+    oa._check(lib.opiv_run_pair_async(plan.handle, buf.ptr, db.ptr, out.ptr, plan.count))
+    gpu_ctx.synchronize()
