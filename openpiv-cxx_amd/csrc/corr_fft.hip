@@ -274,8 +274,8 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     // ---- forward transforms; with two rounds the spectra pack as
     // Q = P0 + i 2^k P1, whose inverse is c0 + i 2^k c1 (both planes are
     // real).  The second window's columns are loaded after the first
-    // window's spectrum: loading them earlier would cost the registers of a
-    // third wave per SIMD at 32 x 32. ----
+    // window's half-row product: loading them earlier would cost the
+    // registers of the fourth wave per SIMD at 32 x 32. ----
     float2 q[W];
     float inv_scale = 1.f;
     const bool compat = p.real_compat != 0;
