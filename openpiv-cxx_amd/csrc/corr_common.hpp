@@ -235,10 +235,10 @@ __device__ __forceinline__ int group_min(int v)
 // t: strict 4-neighbour maxima on rows [hlo, hhi] and columns [wlo, whi]
 // (image_utils.impl.h:48-64), reduced to the largest value (and its scan
 // index h*W + t; equal values keep scan order) and the second largest value
-// over the W lanes of the window.
+// and a scan index of it over the W lanes of the window.
 template <int W>
 __device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, int hhi, bool col_ok,
-                                          float& v1, int& i1, float& v2)
+                                          float& v1, int& i1, float& v2, int& i2)
 {
     v1 = -INFINITY;
     v2 = -INFINITY;
@@ -259,15 +259,33 @@ __device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, i
     // keeps the multiply out of the selects above (folded in, it turns every
     // h into a non-inline constant held in a register)
     asm("" : "+v"(h1));
-    i1 = h1 * W + t;
+    const int li = h1 * W + t;                    // this lane's best as a scan index
     // top 2 of the group: the largest value; among the lanes holding it the
     // smallest index (scan order; indices of different lanes differ); then
     // the largest of that lane's second value and every other lane's first
     // (a tie with another lane's first gives v2 == v1, as the sort does)
     const float gv1 = group_max<W>(v1);
-    i1 = group_min<W>(v1 == gv1 ? i1 : INT_MAX);
-    v2 = group_max<W>(i1 == h1 * W + t ? v2 : v1);
+    i1 = group_min<W>(v1 == gv1 ? li : INT_MAX);
+    const bool winner = i1 == li;
+    const float cand = winner ? v2 : v1;
+    const float gv2 = group_max<W>(cand);
+    // where the second peak is (for its own near-tie test): a lane holding
+    // it as its best knows its row; the winning lane's second value is
+    // searched for in its column, which only waves that need it do
+    constexpr int kSearch = INT_MAX - 1;
+    i2 = group_min<W>(cand == gv2 ? (winner ? kSearch : li) : INT_MAX);
+    if (__builtin_amdgcn_ballot_w64(i2 == kSearch) != 0) {
+        int hs = 0x7fff;
+        sfor<W - 3>([&](auto Hm) {
+            constexpr int h = W - 3 - decltype(Hm)::value;   // descending: the first in scan order wins
+            const bool inreg = col_ok & (h >= hlo) & (h <= hhi);
+            hs = (inreg & (s[h] == gv2) & (h != h1)) ? h : hs;
+        });
+        const int found = group_min<W>(winner && hs != 0x7fff ? hs * W + t : INT_MAX);
+        i2 = i2 == kSearch ? found : i2;
+    }
     v1 = gv1;
+    v2 = gv2;
 }
 
 // Results of the ROUNDS windows of this lane group from their swapped planes
@@ -310,14 +328,19 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
 
         float v1[ROUNDS], v2[ROUNDS];
         int px[ROUNDS], py[ROUNDS];
-        bool two[ROUNDS];
+        int px2[ROUNDS], py2[ROUNDS];          // the second peak (1, 1 when unknown)
+        bool two[ROUNDS], at2[ROUNDS];
         sfor<ROUNDS>([&](auto R) {
             constexpr int r = decltype(R)::value;
             int i1;
-            scan_top2<W>(s[r], t, hlo, hhi, col_ok, v1[r], i1, v2[r]);
+            int i2;
+            scan_top2<W>(s[r], t, hlo, hhi, col_ok, v1[r], i1, v2[r], i2);
             two[r] = v2[r] > -INFINITY;
             px[r] = two[r] ? (i1 & (W - 1)) : 1;
             py[r] = two[r] ? (i1 / W) : 1;
+            at2[r] = two[r] & (i2 < W * W);
+            px2[r] = at2[r] ? (i2 & (W - 1)) : 1;
+            py2[r] = at2[r] ? (i2 / W) : 1;
         });
         // stencil value j of round r: (l, c, r, up, down) = plane positions
         // (py, px - 1), (py, px), (py, px + 1), (py - 1, px), (py + 1, px)
@@ -339,6 +362,19 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
         constexpr int NP = (NV + W - 1) / W;
         float sv[NP];          // the stencil value whose log this lane takes
         float nb[4];           // lanes t < ROUNDS: the four neighbours of round t's peak
+        float nb2[4];          // and of its second peak
+        // neighbour j (left, right, up, down) of round r's second peak
+        auto around2 = [&](int r, int j) -> float {
+            int cx = px2[0], cy = py2[0];
+            sfor<ROUNDS - 1>([&](auto R) {
+                constexpr int q = decltype(R)::value + 1;
+                cx = r == q ? px2[q] : cx;
+                cy = r == q ? py2[q] : cy;
+            });
+            cx += j == 0 ? -1 : (j == 1 ? 1 : 0);
+            cy += j == 2 ? -1 : (j == 3 ? 1 : 0);
+            return fetch(r, cy, cx);
+        };
         if constexpr (SEQ) {
             sfor<ROUNDS>([&](auto R) {
                 constexpr int r = decltype(R)::value;
@@ -358,6 +394,11 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
                     nb[1] = at(2);
                     nb[2] = at(3);
                     nb[3] = at(4);
+                    sfor<4>([&](auto J) {
+                        constexpr int j = decltype(J)::value;
+                        nb2[j] = fetch(r, py2[r] + (j == 2 ? -1 : (j == 3 ? 1 : 0)),
+                                       px2[r] + (j == 0 ? -1 : (j == 1 ? 1 : 0)));
+                    });
                 }
             });
         } else {
@@ -404,13 +445,14 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
                 ln[j] = v;
             });
         }
-        bool act[ROUNDS], tw[ROUNDS];
+        bool act[ROUNDS], tw[ROUNDS], kn2[ROUNDS];
         int gx0[ROUNDS], gy0[ROUNDS], gout[ROUNDS];
         uint32_t gh[ROUNDS];
         sfor<ROUNDS>([&](auto R) {
             constexpr int r = decltype(R)::value;
             act[r] = active[r];
             tw[r] = two[r];
+            kn2[r] = at2[r];
             gx0[r] = geo[r].x0;
             gy0[r] = geo[r].y0;
             gout[r] = geo[r].out;
@@ -424,8 +466,10 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
             nb[1] = stencil(rr, 2);
             nb[2] = stencil(rr, 3);
             nb[3] = stencil(rr, 4);
+            sfor<4>([&](auto J) { nb2[decltype(J)::value] = around2(rr, decltype(J)::value); });
         }
         const float nmax = us * fmaxf(fmaxf(nb[0], nb[1]), fmaxf(nb[2], nb[3]));
+        const float nmax2 = us * fmaxf(fmaxf(nb2[0], nb2[1]), fmaxf(nb2[2], nb2[3]));
         const float fv1 = us * pick(v1), fv2 = us * pick(v2);
         const int fpx = pick(px), fpy = pick(py);
         const int x0 = pick(gx0), y0 = pick(gy0);
@@ -447,7 +491,11 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
             uint32_t flags = 0;
             if (!isfinite(plx) || !isfinite(ply)) flags |= OPIV_FLAG_NONFINITE_FIT;
             const float tol = p.tie_tol * fabsf(fv1);
-            if (fv1 - fv2 <= tol || fv1 - nmax <= tol) flags |= OPIV_FLAG_NEAR_TIE;
+            // the order of the top two, or either one's strict-maximum test,
+            // within FP32 reach: the second peak's test decides whether the
+            // window has two peaks at all
+            if (fv1 - fv2 <= tol || fv1 - nmax <= tol || (pick(kn2) && fv2 - nmax2 <= tol))
+                flags |= OPIV_FLAG_NEAR_TIE;
             if (fv1 == fv2) flags |= OPIV_FLAG_EXACT_TIE;
             // main.cpp:220-234
             const int midx = x0 + W / 2, midy = y0 + W / 2;
