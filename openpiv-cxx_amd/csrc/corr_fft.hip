@@ -455,9 +455,8 @@ static hipError_t launch_w(const CorrParams& p, bool planes, int nt, int blocks,
         if (nt != 256) return hipErrorInvalidValue;
         // one task per wave; at 64 x 64 (LDS-bound at two waves per SIMD)
         // one wave per block, so that a finished wave frees its LDS without
-        // waiting for the block's others.  (Its two-window instance spills
-        // ~20 VGPRs and the 256-thread one none, yet it runs as fast for f32
-        // frames, 6.5% faster for u16, and faster in batches.)
+        // waiting for the block's others (6.5% faster than 256 threads for
+        // u16 frames, as fast for f32, faster in batches).
         if constexpr (W == 64) return launch_wn<W, DT, 64, false>(p, planes, 0, stream);
         else return launch_wn<W, DT, 256, false>(p, planes, 0, stream);
     }
@@ -467,6 +466,11 @@ static hipError_t launch_w(const CorrParams& p, bool planes, int nt, int blocks,
     return hipErrorInvalidValue;
 }
 
+#ifndef OPIV_CORR_FFT_W64_ONLY
+// the 64 x 64 launches for f32 frames, compiled in corr_fft64.hip
+template <int DT>
+hipError_t launch_w64(const CorrParams& p, bool planes, int nt, int blocks, hipStream_t stream);
+
 template <int DT>
 static hipError_t launch_dt(const CorrParams& p, int win, bool planes, int nt, int blocks, hipStream_t stream)
 {
@@ -474,7 +478,9 @@ static hipError_t launch_dt(const CorrParams& p, int win, bool planes, int nt, i
     case 8: return launch_w<8, DT>(p, planes, nt, blocks, stream);
     case 16: return launch_w<16, DT>(p, planes, nt, blocks, stream);
     case 32: return launch_w<32, DT>(p, planes, nt, blocks, stream);
-    case 64: return launch_w<64, DT>(p, planes, nt, blocks, stream);
+    case 64:
+        if constexpr (DT == 2) return launch_w64<DT>(p, planes, nt, blocks, stream);
+        else return launch_w<64, DT>(p, planes, nt, blocks, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -496,5 +502,6 @@ hipError_t launch_corr_fft(const CorrParams& p, int win, int dtype, bool planes,
 {
     return launch_corr_fft_shape(p, win, dtype, planes, 256, 0, stream);
 }
+#endif // OPIV_CORR_FFT_W64_ONLY
 
 } // namespace opiv
