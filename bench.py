@@ -40,7 +40,11 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 I8_MFMA_PEAK_TOPS = 5000.0
 METRIC = "vectors/sec at 32² win 50% overlap, chosen vs plain GPU path at ≤1e-3 px"
-RESULT_BYTES = 72
+RESULT_BYTES = 72          # one opiv_window_result as stored
+# algorithmic result bytes per window: the vector (u, v), its position and
+# s/n as the reference's output needs them, 16 B (the record itself is 72 B)
+ALG_RESULT_BYTES = 16
+PREWARM_S = 0.3            # untimed launches before the warm-up steps
 
 
 def log(*a):
@@ -119,34 +123,69 @@ def workload(win: int, dtype: str = "f32"):
     return a, b, w, step
 
 
-def cpu_baseline(a, b, win, step, threads):
-    """Reference FP64 path on host cores: vectors/s over the whole pair."""
+def host_cores():
+    """Cores this process may run on (affinity mask) and the CPU model."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return n, model
+
+
+def cpu_loop_times(a, b, win, step, threads, timed=5):
+    """Processing-loop times (s) of the reference's FP64 path on `threads`
+    pool threads: one warm-up loop, then `timed` loops (main.cpp:239,304-309
+    time only the loop).  Returns (times, kind)."""
     ref_bin = ROOT / "oracle" / "_ref" / "ref_piv"
     h, w = a.shape
-    nwin = (1 + (w - win) // step) * (1 + (h - win) // step)
     if ref_bin.exists():
         with tempfile.TemporaryDirectory() as d:
             pa, pb = Path(d) / "a.bin", Path(d) / "b.bin"
             a.astype(np.float64).tofile(pa)
             b.astype(np.float64).tofile(pb)
-            reps = 4
             out = subprocess.run([str(ref_bin), "time", str(pa), str(pb), str(w), str(h), str(win), str(step),
-                                  str(step), str(threads), str(reps)], capture_output=True, text=True,
+                                  str(step), str(threads), str(timed + 1)], capture_output=True, text=True,
                                  check=True, timeout=600).stdout.split("\n")
-            us = sorted(float(l.split()[1]) for l in out if l.strip())[1:]   # drop the slowest (warm-up)
-        t = float(np.median(us)) * 1e-6
-        kind = "reference"
-        sample = (f"whole {w}x{h} pair, {nwin} windows, reference FP64 algos::FFT path + ThreadPool "
-                  f"'pool' strategy (examples/process/main.cpp:189-276), median of {len(us)} timed loops")
-    else:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle as orc
+        # lines in run order; the first loop is the warm-up
+        us = [float(l.split()[1]) for l in out if l.strip()][1:]
+        return [u * 1e-6 for u in us], "reference"
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    times = []
+    for k in range(timed + 1):
         t0 = time.perf_counter()
         orc.process_pair(a, b, win, step, step, threads=threads)
-        t = time.perf_counter() - t0
-        kind = "port"
-        sample = f"whole {w}x{h} pair, {nwin} windows, FP64 oracle restatement, one timed loop"
-    return {"value": nwin / t, "unit": "vectors/s", "cores": threads, "kind": kind, "sample": sample}
+        if k:
+            times.append(time.perf_counter() - t0)
+    return times, "port"
+
+
+def cpu_baseline(a, b, win, step):
+    """Reference FP64 path on this host's cores, vectors/s over the whole
+    pair: nproc - 1 pool threads (the reference default, main.cpp:46) and
+    nproc threads, median of 5 loops after 1 warm-up each.  `value` is the
+    nproc - 1 figure."""
+    h, w = a.shape
+    nwin = (1 + (w - win) // step) * (1 + (h - win) // step)
+    ncores, model = host_cores()
+    by_threads = {}
+    kind = "port"
+    for threads in sorted({max(1, ncores - 1), ncores}):
+        log(f"[bench] cpu baseline, {threads} threads")
+        times, kind = cpu_loop_times(a, b, win, step, threads)
+        by_threads[str(threads)] = round(nwin / float(np.median(times)), 1)
+    default_threads = max(1, ncores - 1)
+    what = ("reference FP64 algos::FFT path + its ThreadPool 'pool' strategy (examples/process/main.cpp:189-276), "
+            "built from the reference sources") if kind == "reference" else "FP64 oracle restatement"
+    return {"value": by_threads[str(default_threads)], "unit": "vectors/s", "cores": default_threads, "kind": kind,
+            "sample": f"whole {w}x{h} pair, {nwin} windows, {what}; median of 5 timed loops after 1 warm-up",
+            "by_threads": by_threads, "host_cores": ncores, "cpu_model": model}
 
 
 def accuracy(res, a, b, win, step, sample=4096, seed=0):
@@ -373,7 +412,7 @@ def bench_batch(steps: int, warmup: int):
                        "pairs": len(pairs), "pixels_per_frame_total": npx, "windows": batch.count,
                        "windows_per_pass": per_pass,
                        "shapes": [batch.shape(p) for p in range(3)], "create_and_tune_s": round(tune_s, 2)},
-            "roofline": {"bound": "hbm", "achieved": round((3 * 2 * 4 * npx + batch.count * RESULT_BYTES)
+            "roofline": {"bound": "hbm", "achieved": round((3 * 2 * 4 * npx + batch.count * ALG_RESULT_BYTES)
                                                            / (ms * 1e-3) / 1e9, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "note": "algorithmic bytes: every pass reads both frames once"},
@@ -453,6 +492,14 @@ def main():
         plan = oa.Plan(ctx, w, w, win, step=(step, step), dtype=np_dtype, variant=variant)
         n = plan.count
         dout = oa.DeviceBuffer(n * RESULT_BYTES)
+        # clock settling: an idle GPU ramps its clocks up over the first tens
+        # of milliseconds of work, so the launches run for PREWARM_S of wall
+        # time before the W warm-up steps (none of it is timed)
+        t_pre = time.perf_counter()
+        while time.perf_counter() - t_pre < PREWARM_S:
+            for _ in range(20):
+                plan.run_async(da.ptr, db.ptr, dout.ptr)
+            ctx.synchronize()
         for _ in range(args.warmup):
             plan.run_async(da.ptr, db.ptr, dout.ptr)
         ctx.synchronize()
@@ -485,7 +532,7 @@ def main():
     acc = accuracy(res, a, b, win, step)
 
     frame_bytes = 2 * w * w * np.dtype(np_dtype).itemsize
-    alg_bytes = frame_bytes + n * RESULT_BYTES
+    alg_bytes = frame_bytes + n * ALG_RESULT_BYTES
     achieved = alg_bytes / (ms_per_step * 1e-3) / 1e9
     # measured traffic exists for the default grid step (half the window) only
     key = f"{'direct_bf16' if chosen == 'direct' else 'fft_fp32'}_w{win}_4096_{args.dtype}"
@@ -493,6 +540,8 @@ def main():
         key += f"_s{step}"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(key),
+                "traffic_calibration": "rocprofv3 --pmc FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (KiB per dispatch; "
+                                       "factors from a device-to-device copy of known size)",
                 "algorithmic_bytes_per_launch": alg_bytes,
                 "bytes_per_window": round(alg_bytes / n, 1)}
 
@@ -521,9 +570,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(16, (os.cpu_count() or 2)) - 1)
-        log(f"[bench] cpu baseline, {threads} threads")
-        cpu = cpu_baseline(a, b, win, step, threads)
+        cpu = cpu_baseline(a, b, win, step)
 
     direct_i8 = chosen == "direct" and np_dtype == np.uint8
     path_name = {"fft": "fft_fp32_fused",
@@ -542,7 +589,7 @@ def main():
         "plain_gpu": {"path": "fft_fp32_fused", "ms_per_step": round(plain_ms, 5),
                       "value": round(n / (plain_ms * 1e-3), 1)},
         "roofline": roofline, "compute_roofline": compute, "cpu_baseline": cpu, "accuracy": acc,
-        "wall_s_timed_region": round(wall, 4),
+        "wall_s_timed_region": round(wall, 4), "prewarm_s": PREWARM_S,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -170,11 +170,15 @@ int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t di
  *
  * loaders/image_loader.cpp:23-57 (format from the first bytes),
  * pnm_image_loader.cpp:279-310 and tiff_image_loader.cpp:160-236 (read
- * without libtiff): PNM P5 and baseline TIFF (uncompressed strips, one
- * sample of 8 or 16 bits, every page), delivered in their own pixel type
- * (OPIV_U8 / OPIV_U16) -- no widening.  Errors keep the reference's texts
- * ("failed to open ...", "failed to find loader for ...", "image index out
- * of range ..."). */
+ * without libtiff): PNM P5 (grey) and P6 (RGB); TIFF strips of one or three
+ * samples of 8 or 16 bits, uncompressed, Deflate, LZW or PackBits (the
+ * horizontal predictor with LZW and Deflate), every page.  RGB becomes grey
+ * by the reference's integer rule (218 r + 732 g + 74 b) >> 10
+ * (pixel_types.h:397-400).  Frames are delivered in their own depth
+ * (OPIV_U8 / OPIV_U16; 8-bit P6 as OPIV_U16, since the reference reads its
+ * samples through a signed char) -- no widening.  Errors keep the
+ * reference's texts ("failed to open ...", "failed to find loader for ...",
+ * "image index out of range ..."). */
 typedef struct opiv_image_info {
     uint32_t width, height;
     int32_t dtype;           /* OPIV_U8 or OPIV_U16 */

@@ -24,6 +24,7 @@
 #include "corr_fft.hpp"
 #include "ctx.hpp"
 #include "openpiv_amd.h"
+#include "plan_check.hpp"
 
 using namespace opiv::detail;
 
@@ -128,37 +129,18 @@ int fill_table(opiv_batch* b, const opiv_pair_desc* pairs, size_t npairs)
     size_t total = 0;
     for (uint32_t ps = 0; ps < d.npasses; ++ps) {
         const uint32_t win = d.win[ps];
-        uint32_t ox, oy;
-        int rc = opiv_grid_offsets(win, win, d.overlap[ps], &ox, &oy);
-        if (rc) return rc;
+        int rc = OPIV_OK;
         int64_t first = 0;
         const int64_t align = opiv::batch_align((int)win, (int)d.dtype);
         const size_t pass_base = total;
         for (size_t i = 0; i < npairs; ++i) {
             const auto& pr = pairs[i];
-            if (!pr.a || !pr.b) return fail(OPIV_E_INVALID, "pair %zu: null frame pointer", i);
-            if (((uintptr_t)pr.a | (uintptr_t)pr.b) % dtype_size(d.dtype) != 0)
-                return fail(OPIV_E_INVALID, "pair %zu: frame pointers must be aligned to their %zu-byte pixels", i,
-                            dtype_size(d.dtype));
-            const size_t pitch = pr.pitch ? pr.pitch : pr.img_w;
-            if (pitch < pr.img_w) return fail(OPIV_E_INVALID, "pair %zu: pitch smaller than width", i);
-            if (pitch * dtype_size(d.dtype) > opiv::kMaxPitchBytes)
-                return fail(OPIV_E_UNSUPPORTED, "pair %zu: row pitch of %zu bytes exceeds %zu", i,
-                            pitch * dtype_size(d.dtype), opiv::kMaxPitchBytes);
-            uint32_t nx, ny;
-            rc = opiv_grid(pr.img_w, pr.img_h, win, win, ox, oy, &nx, &ny, nullptr);
-            if (rc) return fail(rc, "pair %zu, pass %u: %s", i, ps, opiv_last_error());
             auto& e = b->table[(size_t)ps * npairs + i];
-            e.a = pr.a;
-            e.b = pr.b;
-            e.pitch = pitch;
-            e.x_start = (int32_t)((pr.img_w - win - ox * (nx - 1)) / 2);
-            e.y_start = (int32_t)((pr.img_h - win - oy * (ny - 1)) / 2);
-            e.step_x = (int32_t)ox;
-            e.step_y = (int32_t)oy;
-            e.nx = (int32_t)nx;
-            e.ny = (int32_t)ny;
-            e.img_h = pr.img_h;
+            rc = check_batch_pair(pr, i, ps, win, d.overlap[ps], (int)d.dtype, &e);
+            if (rc) return rc;
+            if (ps == 0 && (!is_device_ptr(pr.a) || !is_device_ptr(pr.b)))
+                return fail(OPIV_E_INVALID, "pair %zu: frames must be device memory", i);
+            const uint32_t nx = (uint32_t)e.nx, ny = (uint32_t)e.ny;
             e.first = (int32_t)first;
             e.out_first = (int32_t)(total - pass_base);
             b->offsets[(size_t)ps * npairs + i] = total;
@@ -237,11 +219,8 @@ int opiv_batch_create(opiv_ctx* ctx, const opiv_batch_desc* desc, const opiv_pai
     if (dtype_size(desc->dtype) == 0) return fail(OPIV_E_INVALID, "unknown dtype %d", desc->dtype);
     if (desc->variant != OPIV_CIRCULAR_FFT_FP32 && desc->variant != OPIV_REAL_COMPAT)
         return fail(OPIV_E_UNSUPPORTED, "unknown variant %d", desc->variant);
-    for (uint32_t ps = 0; ps < desc->npasses; ++ps) {
-        const uint32_t w = desc->win[ps];
-        if (!is_pow2(w)) return fail(OPIV_E_NOT_POW2, "dimensions must be power of 2: [%u,%u]", w, w);
-        if (w < 8 || w > 64) return fail(OPIV_E_UNSUPPORTED, "window %u outside the supported range [8, 64]", w);
-    }
+    for (uint32_t ps = 0; ps < desc->npasses; ++ps)
+        if (int rc = check_window(desc->win[ps])) return rc;
     if (npairs > 0x7fffffffull) return fail(OPIV_E_UNSUPPORTED, "too many pairs");
     HIP_TRY(hipSetDevice(ctx->device));
     opiv_batch* b = new opiv_batch;
@@ -335,6 +314,8 @@ int opiv_batch_run(opiv_batch* batch, opiv_window_result* d_out, size_t n)
         return fail(OPIV_E_SIZE, "result count %zu differs from the batch's %zu windows", n, batch->offsets.back());
     opiv_ctx* ctx = batch->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
+    if (batch->graph_out != d_out && !is_device_ptr(d_out))
+        return fail(OPIV_E_INVALID, "opiv_batch_run: results must be device memory");
     if (!batch->exec || batch->graph_out != d_out) {
         drop_graph(batch);
         HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));

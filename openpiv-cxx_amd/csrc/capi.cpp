@@ -17,6 +17,7 @@
 #include "corr_fft.hpp"
 #include "ctx.hpp"
 #include "openpiv_amd.h"
+#include "plan_check.hpp"
 #include "transform.hpp"
 
 using namespace opiv::detail;
@@ -80,104 +81,23 @@ int opiv_ctx_synchronize(opiv_ctx* ctx)
 
 void* opiv_ctx_stream(opiv_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
-int opiv_grid_offsets(uint32_t win_w, uint32_t win_h, double overlap, uint32_t* off_x, uint32_t* off_y)
-{
-    if (!off_x || !off_y) return fail(OPIV_E_INVALID, "opiv_grid_offsets: null output pointer");
-    // grid.impl.h:14-21
-    if (overlap < 0.0 || overlap > 1.0) return fail(OPIV_E_GRID, "offsets must be between 0.0 and 1.0");
-    *off_x = (uint32_t)(win_w * overlap);
-    *off_y = (uint32_t)(win_h * overlap);
-    return OPIV_OK;
-}
-
-int opiv_grid(uint32_t img_w, uint32_t img_h, uint32_t win_w, uint32_t win_h,
-              uint32_t off_x, uint32_t off_y, uint32_t* nx, uint32_t* ny, int32_t* xy)
-{
-    if (!nx || !ny) return fail(OPIV_E_INVALID, "opiv_grid: null output pointer");
-    // grid.impl.h:29-40 (size::area() is a uint32 product)
-    if ((uint32_t)(img_w * img_h) == 0) return fail(OPIV_E_GRID, "image size must be non-zero");
-    if ((uint32_t)(win_w * win_h) == 0) return fail(OPIV_E_GRID, "interrogation size must be non-zero");
-    if (off_x == 0 || off_y == 0) return fail(OPIV_E_GRID, "offsets must be non-zero");
-    if (win_w > img_w || win_h > img_h) return fail(OPIV_E_GRID, "interrogation size is bigger than image");
-    // grid.impl.h:44-58
-    const uint32_t x_count = 1 + (img_w - win_w) / off_x;
-    const uint32_t y_count = 1 + (img_h - win_h) / off_y;
-    const uint32_t x_start = (img_w - win_w - off_x * (x_count - 1)) / 2;
-    const uint32_t y_start = (img_h - win_h - off_y * (y_count - 1)) / 2;
-    *nx = x_count;
-    *ny = y_count;
-    if (xy) {
-        size_t k = 0;
-        for (uint32_t y = 0; y < y_count; ++y)
-            for (uint32_t x = 0; x < x_count; ++x, ++k) {
-                xy[2 * k] = (int32_t)(x_start + x * off_x);
-                xy[2 * k + 1] = (int32_t)(y_start + y * off_y);
-            }
-    }
-    return OPIV_OK;
-}
-
-// OPIV_AUTO: the measured faster path per window size (both compute the
-// same circular correlation): the MFMA direct path for 16 x 16 windows, the
-// fused FFT otherwise
-// OPIV_AUTO picks the faster path measured on MI355X (kernel traces, 4096^2
-// pairs, 16 x 16 windows at step 8): for 8-bit frames the direct MFMA kernel
-// (105.6 us, and its correlation is exact) against the FFT kernel's 109.4 us;
-// for 16-bit and float frames the FFT kernel (110 / 109 us against 200 / 312
-// us direct).  Other window sizes have no direct kernel.
-// OPIV_AUTO: the faster path as measured on MI355X at 4096 x 4096: the direct
-// kernel for 8-bit frames at 16 x 16 (exact int8 MFMA, 74 us against the FFT
-// kernel's 83 us), the FFT kernel otherwise (16 x 16 u16: 90 against 198 us,
-// f32: 89 against 309 us).
-static int32_t auto_variant(int win, int dtype)
-{
-    return opiv::direct_supported(win) && dtype == OPIV_U8 ? OPIV_DIRECT_BF16_SPLIT : OPIV_CIRCULAR_FFT_FP32;
-}
-
 int opiv_plan_create(opiv_ctx* ctx, const opiv_plan_desc* desc, opiv_plan** out)
 {
     if (!ctx || !desc || !out) return fail(OPIV_E_INVALID, "opiv_plan_create: null argument");
     *out = nullptr;
-    const uint32_t win = desc->win;
-    // fft.h:82-83
-    if (!is_pow2(win)) return fail(OPIV_E_NOT_POW2, "dimensions must be power of 2: [%u,%u]", win, win);
-    if (win < 8 || win > 64)
-        return fail(OPIV_E_UNSUPPORTED, "window %u outside the supported range [8, 64]", win);
-    if (dtype_size(desc->dtype) == 0) return fail(OPIV_E_INVALID, "unknown dtype %d", desc->dtype);
-    if (desc->variant < OPIV_CIRCULAR_FFT_FP32 || desc->variant > OPIV_AUTO)
-        return fail(OPIV_E_UNSUPPORTED, "unknown variant %d", desc->variant);
-    if (desc->variant == OPIV_DIRECT_BF16_SPLIT && !opiv::direct_supported((int)win))
-        return fail(OPIV_E_UNSUPPORTED, "the direct BF16 path supports 16x16 windows, not %u", win);
-    const size_t pitch = desc->pitch ? desc->pitch : desc->img_w;
-    if (pitch < desc->img_w) return fail(OPIV_E_INVALID, "pitch %zu smaller than width %u", pitch, desc->img_w);
-    if (pitch * dtype_size(desc->dtype) > opiv::kMaxPitchBytes)
-        return fail(OPIV_E_UNSUPPORTED, "row pitch of %zu bytes exceeds %zu", pitch * dtype_size(desc->dtype),
-                    opiv::kMaxPitchBytes);
-
+    PlanGeom g;
+    if (int rc = check_plan_desc(desc, &g)) return rc;
     opiv_plan* p = new opiv_plan;
     p->ctx = ctx;
     p->desc = *desc;
-    p->desc.pitch = pitch;
-    if (desc->variant == OPIV_AUTO) p->desc.variant = auto_variant((int)win, desc->dtype);
-    int rc = OPIV_OK;
-    if (desc->step_x && desc->step_y) {
-        p->off_x = desc->step_x;
-        p->off_y = desc->step_y;
-    } else {
-        rc = opiv_grid_offsets(win, win, desc->overlap, &p->off_x, &p->off_y);
-    }
-    if (rc == OPIV_OK)
-        rc = opiv_grid(desc->img_w, desc->img_h, win, win, p->off_x, p->off_y, &p->nx, &p->ny, nullptr);
-    if (rc != OPIV_OK) {
-        delete p;
-        return rc;
-    }
-    if ((uint64_t)p->nx * p->ny > 0x7fffffffull) {
-        delete p;
-        return fail(OPIV_E_UNSUPPORTED, "too many windows");
-    }
-    p->x_start = (int32_t)((desc->img_w - win - p->off_x * (p->nx - 1)) / 2);
-    p->y_start = (int32_t)((desc->img_h - win - p->off_y * (p->ny - 1)) / 2);
+    p->desc.pitch = g.pitch;
+    p->desc.variant = g.variant;
+    p->off_x = g.off_x;
+    p->off_y = g.off_y;
+    p->nx = g.nx;
+    p->ny = g.ny;
+    p->x_start = g.x_start;
+    p->y_start = g.y_start;
     *out = p;
     return OPIV_OK;
 }
@@ -249,6 +169,8 @@ int opiv_run_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, opiv_
     const size_t count = (size_t)plan->nx * plan->ny;
     if (n != count) return fail(OPIV_E_SIZE, "result count %zu differs from the grid's %zu windows", n, count);
     if (int rc = check_frames(plan, d_a, d_b, "opiv_run_pair_async")) return rc;
+    if (!is_device_ptr(d_a) || !is_device_ptr(d_b) || !is_device_ptr(d_out))
+        return fail(OPIV_E_INVALID, "opiv_run_pair_async: frames and results must be device memory");
     HIP_TRY(hipSetDevice(plan->ctx->device));
     opiv::CorrParams p = plan_params(plan, d_a, d_b, d_out);
     HIP_TRY(launch_plan(plan, p, plan->ctx->stream));
@@ -294,13 +216,12 @@ static int correlate_stacked(opiv_ctx* ctx, uint32_t win, size_t n, int variant,
                              const double* a, const double* b, double* planes)
 {
     if (!ctx || !a || !b || !planes) return fail(OPIV_E_INVALID, "correlate: null argument");
-    if (!is_pow2(win)) return fail(OPIV_E_NOT_POW2, "dimensions must be power of 2: [%u,%u]", win, win);
-    if (win < 8 || win > 64) return fail(OPIV_E_UNSUPPORTED, "window %u outside the supported range [8, 64]", win);
+    if (int rc = check_window(win)) return rc;
     if (variant < OPIV_CIRCULAR_FFT_FP32 || variant > OPIV_AUTO)
         return fail(OPIV_E_UNSUPPORTED, "unknown variant %d", variant);
     if (variant == OPIV_AUTO) variant = auto_variant((int)win, OPIV_F64);
     if (variant == OPIV_DIRECT_BF16_SPLIT && !opiv::direct_supported((int)win))
-        return fail(OPIV_E_UNSUPPORTED, "the direct BF16 path supports 16x16 windows, not %u", win);
+        return fail(OPIV_E_UNSUPPORTED, "the direct path does not implement %ux%u windows", win, win);
     if (n == 0) return OPIV_OK;
     if (n > 0x7fffffffull / win) return fail(OPIV_E_UNSUPPORTED, "too many windows");
     HIP_TRY(hipSetDevice(ctx->device));
@@ -435,6 +356,10 @@ int opiv_vjp_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, const
     if (plan->desc.variant == OPIV_REAL_COMPAT)
         return fail(OPIV_E_UNSUPPORTED, "the adjoint supports the circular correlation only");
     if (int rc = check_frames(plan, d_a, d_b, "opiv_vjp_pair_async")) return rc;
+    for (const void* q : { d_a, d_b, (const void*)d_fwd, (const void*)d_cot, (const void*)d_grad_a,
+                           (const void*)d_grad_b })
+        if (!is_device_ptr(q))
+            return fail(OPIV_E_INVALID, "opiv_vjp_pair_async: every buffer must be device memory");
     HIP_TRY(hipSetDevice(plan->ctx->device));
     opiv::AdjParams p = adjoint_params(plan, d_a, d_b, d_fwd);
     return launch_vjp(plan, p, d_cot, d_grad_a, d_grad_b);
@@ -584,8 +509,7 @@ int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t di
                                const double* in, double* out)
 {
     if (!ctx || !in || !out) return fail(OPIV_E_INVALID, "transform: null argument");
-    if (!is_pow2(win)) return fail(OPIV_E_NOT_POW2, "dimensions must be power of 2: [%u,%u]", win, win);
-    if (win < 8 || win > 64) return fail(OPIV_E_UNSUPPORTED, "window %u outside the supported range [8, 64]", win);
+    if (int rc = check_window(win)) return rc;
     if (direction != OPIV_FORWARD && direction != OPIV_REVERSE)
         return fail(OPIV_E_INVALID, "unknown direction %d", direction);
     if (n == 0) return OPIV_OK;

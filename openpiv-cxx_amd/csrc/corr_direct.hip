@@ -386,8 +386,6 @@ static hipError_t launch_direct_dt(const CorrParams& p, bool planes, hipStream_t
     return hipGetLastError();
 }
 
-bool direct_supported(int win) { return win == 16; }
-
 hipError_t launch_corr_direct(const CorrParams& p, int win, int dtype, bool planes, hipStream_t stream)
 {
     if (p.n_windows <= 0) return hipSuccess;

@@ -8,9 +8,6 @@
 
 namespace opiv {
 
-// window sizes the direct path implements
-bool direct_supported(int win);
-
 // same parameters and outputs as launch_corr_fft (single pair, one task per
 // window); the circular variant only
 hipError_t launch_corr_direct(const CorrParams& p, int win, int dtype, bool planes, hipStream_t stream);

@@ -6,38 +6,10 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "geom.hpp"
 #include "openpiv_amd.h"
 
 namespace opiv {
-
-// One frame pair of a batch pass (device table): its frames and its grid.
-// The pass's windows are numbered pair by pair, pair k owning
-// [first, first + nx * ny); each pair's range starts on a multiple of the
-// kernel's task size (batch_align), so every window shares its packed
-// inverse FFT with the same partner as in a single-pair launch.  Results of
-// pair k start at out_first.
-struct BatchPair {
-    const void* a;
-    const void* b;
-    size_t pitch;
-    int32_t x_start, y_start;
-    int32_t step_x, step_y;
-    int32_t nx, ny;
-    uint32_t img_h;
-    int32_t first;
-    int32_t out_first;
-    int32_t pad_;
-};
-
-// windows per wave task of the kernel for window size win and pixel type
-// dtype (0 u8, 1 u16, 2 f32, 3 f64): two rounds of 64 / win lane groups, one
-// round for f64 frames at 64 x 64
-constexpr int batch_align(int win, int dtype) { return (win == 64 && dtype == 3 ? 1 : 2) * (64 / win); }
-
-// Largest frame row pitch in bytes.  The FFT kernel addresses a wave task's
-// pixels as 32-bit byte offsets from the task's first window: a task spans
-// at most 129 rows of pitch, plus 63 more for a window's last row.
-constexpr size_t kMaxPitchBytes = size_t(1) << 24;
 
 // One launch processes windows [first, first + n_windows) of a cartesian
 // grid (grid.impl.h:44-58): window g sits at

@@ -1,32 +1,18 @@
 // ctx.hpp -- host-side internals shared by the C ABI translation units:
-// the per-thread error text, status helpers, growable device buffers and the
-// context and plan objects behind the opaque handles of openpiv_amd.h.
+// HIP status helpers, growable device buffers and the context and plan
+// objects behind the opaque handles of openpiv_amd.h.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
-#include <cstdarg>
 #include <cstdint>
-#include <cstdio>
 #include <cstring>
 #include <string>
 
 #include "openpiv_amd.h"
+#include "status.hpp"
 
 namespace opiv::detail {
-
-inline thread_local std::string g_last_error;
-
-inline int fail(int code, const char* fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_last_error = buf;
-    return code;
-}
 
 inline int hip_fail(hipError_t e, const char* what)
 {
@@ -38,23 +24,6 @@ inline int hip_fail(hipError_t e, const char* what)
         hipError_t e_ = (call);                              \
         if (e_ != hipSuccess) return hip_fail(e_, #call);    \
     } while (0)
-
-inline bool is_pow2(uint64_t v) { return v && (v == (v & (~v + 1))); }   // util.h:67-70
-
-inline size_t dtype_size(int dt)
-{
-    switch (dt) {
-    case OPIV_U8: return 1;
-    case OPIV_U16: return 2;
-    case OPIV_F32: return 4;
-    case OPIV_F64: return 8;
-    default: return 0;
-    }
-}
-
-// relative gap under which two FP32 correlation values count as tied for
-// OPIV_FLAG_NEAR_TIE
-constexpr float kTieTol = 2e-6f;
 
 // device buffer that grows on demand
 struct DevBuf {

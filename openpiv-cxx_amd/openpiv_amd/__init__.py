@@ -367,13 +367,31 @@ class Plan:
         _check(self._lib.opiv_plan_grid(self.handle, xy.ctypes.data))
         return xy
 
+    def _host_frames(self, a, b):
+        """Host frames as contiguous arrays of the plan's pixel type, each
+        holding at least pitch * img_h pixels (the C ABI reads that many)."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        b = np.ascontiguousarray(b, dtype=self.dtype)
+        need = (self.desc.pitch or self.desc.img_w) * self.desc.img_h
+        if a.size != b.size or a.size < need:
+            raise ValueError(f"frames of {a.size} and {b.size} pixels; the plan reads {need} "
+                             f"(pitch {self.desc.pitch or self.desc.img_w} x height {self.desc.img_h})")
+        return a, b
+
+    def _host_results(self, out, what="out"):
+        if not isinstance(out, np.ndarray) or out.dtype != RESULT_DTYPE or out.size != self.count \
+                or not out.flags.c_contiguous:
+            raise ValueError(f"{what} must be a contiguous array of {self.count} RESULT_DTYPE records")
+        return out
+
     def run(self, a, b, out=None) -> np.ndarray:
         """Process one frame pair (host numpy arrays or device tensors); returns results."""
-        if isinstance(a, np.ndarray):
-            a = np.ascontiguousarray(a, dtype=self.dtype)
-            b = np.ascontiguousarray(b, dtype=self.dtype)
+        if isinstance(a, np.ndarray) or isinstance(b, np.ndarray):
+            a, b = self._host_frames(a, b)
         if out is None:
             out = np.zeros(self.count, dtype=RESULT_DTYPE)
+        elif isinstance(out, np.ndarray):
+            out = self._host_results(out)
         _check(self._lib.opiv_run_pair(self.handle, _ptr(a), _ptr(b), _ptr(out), self.count))
         return out
 
@@ -383,9 +401,13 @@ class Plan:
 
     def vjp(self, a, b, cot, fwd=None):
         """Frame gradients of sum_i cot[i] . (u_i, v_i): (grad_a, grad_b) float32 (h, pitch)."""
-        a = np.ascontiguousarray(a, dtype=self.dtype)
-        b = np.ascontiguousarray(b, dtype=self.dtype)
-        cot = np.ascontiguousarray(cot, dtype=np.float64).reshape(self.count, 2)
+        a, b = self._host_frames(a, b)
+        if fwd is not None:
+            fwd = self._host_results(fwd, "fwd")
+        cot = np.ascontiguousarray(cot, dtype=np.float64)
+        if cot.size != 2 * self.count:
+            raise ValueError(f"cot must hold 2 x {self.count} values, not {cot.size}")
+        cot = cot.reshape(self.count, 2)
         pitch = self.desc.pitch or self.desc.img_w
         ga = np.zeros((self.desc.img_h, pitch), np.float32)
         gb = np.zeros((self.desc.img_h, pitch), np.float32)
@@ -395,8 +417,9 @@ class Plan:
 
     def variance(self, a, b, sigma: float, fwd=None) -> np.ndarray:
         """(n, 2) variances of (u, v) under i.i.d. pixel noise sigma."""
-        a = np.ascontiguousarray(a, dtype=self.dtype)
-        b = np.ascontiguousarray(b, dtype=self.dtype)
+        a, b = self._host_frames(a, b)
+        if fwd is not None:
+            fwd = self._host_results(fwd, "fwd")
         var = np.zeros((self.count, 2), np.float64)
         _check(self._lib.opiv_variance_pair(self.handle, _ptr(a), _ptr(b), None if fwd is None else _ptr(fwd),
                                             sigma, var.ctypes.data))

@@ -21,6 +21,7 @@
 //   ref_piv gridf IMG_W IMG_H WIN_W WIN_H FRACTION
 //   ref_piv peaks W H NUM RADIUS IN.bin          (prints x y value per peak)
 //   ref_piv time A.bin B.bin W H WIN OFF_X OFF_Y THREADS REPS (prints us per rep)
+//   ref_piv load FILE OUT.bin (the reference's loaders: "w h", FP64 pixels)
 //   ref_piv avgsub OUT_PREFIX F0.pgm F1.pgm ...
 //       examples/average_subtract/main.cpp:98-159 on PNM frames loaded by the
 //       reference's loader: writes OUT_PREFIX<i>.pnm with the reference's PNM
@@ -329,6 +330,25 @@ int cmd_avgsub(int argc, char** argv)
     return 0;
 }
 
+// the reference's loader registry (image_loader.cpp:23-57) reading a file
+// into a gf_image, as examples/process/main.cpp:106-135 loads its frames:
+// prints "w h" and writes the pixels as FP64 to OUT
+int cmd_load(int argc, char** argv)
+{
+    if (argc < 4) return 1;
+    std::ifstream is(argv[2], std::ios::binary);
+    if (!is.is_open()) return 2;
+    auto loader{ core::image_loader_registry::find(is) };
+    if (!loader) return 3;
+    core::gf_image image;
+    if (!loader->load(is, image)) return 4;
+    std::ofstream os(argv[3], std::ios::binary);
+    for (uint32_t h = 0; h < image.height(); ++h)
+        os.write(reinterpret_cast<const char*>(image.line(h)), (std::streamsize)(image.width() * sizeof(double)));
+    std::printf("%u %u\n", image.width(), image.height());
+    return 0;
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -343,5 +363,6 @@ int main(int argc, char** argv)
     if (cmd == "peaks") return cmd_peaks(argc, argv);
     if (cmd == "time") return cmd_time(argc, argv);
     if (cmd == "avgsub") return cmd_avgsub(argc, argv);
+    if (cmd == "load") return cmd_load(argc, argv);
     return 1;
 }
