@@ -351,9 +351,6 @@ def bench_background(steps: int, warmup: int, nframes: int = 16):
     print(json.dumps(line), flush=True)
 
 
-BATCH_SIZES = [(512, 512), (1024, 1024), (2048, 2048), (4096, 4096), (1280, 800)]
-
-
 def fft_flops(win: int) -> float:
     """Algorithmic flops of one window on the FFT path: 5 N log2 N per complex
     N-point transform, halved for real input (two forward transforms, the
@@ -369,22 +366,16 @@ def batch_compute(per_pass, wins, ms):
 
 
 def bench_batch(steps: int, warmup: int):
-    """24 frame pairs with seeded sizes from BATCH_SIZES, each with a seeded
+    """24 frame pairs with seeded sizes from oa.MULTIPASS_SIZES, each with a seeded
     Lamb-Oseen vortex (core radius 40 px, peak 5 px), passes 64 -> 32 -> 16 at
     50% overlap, all windows in one graph launch per step."""
     import openpiv_amd as oa
-    rng = np.random.default_rng(24)
-    picks = rng.integers(0, len(BATCH_SIZES), 24)
     bufs, pairs = [], []
-    log("[bench] generating 24 pairs:", [BATCH_SIZES[k] for k in picks])
-    for i, k in enumerate(picks):
-        w, h = BATCH_SIZES[k]
-        cx, cy = rng.uniform(0.25, 0.75) * w, rng.uniform(0.25, 0.75) * h
-        a, b = oa.synth_pair(w, h, seed=1000 + i, field=oa.FIELD_LAMB_OSEEN, params=(cx, cy, 40.0, 5.0),
-                             dtype=np.float32)
+    for a, b in oa.multipass_pairs(24, seed=24):
         da, db = oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)
         bufs.append((da, db))
-        pairs.append((da.ptr, db.ptr, w, h))
+        pairs.append((da.ptr, db.ptr, a.shape[1], a.shape[0]))
+    log("[bench] 24 pairs:", [(w, h) for (_, _, w, h) in pairs])
     ctx = oa.Context(0)
     t0 = time.perf_counter()
     batch = oa.Batch(ctx, pairs, (64, 32, 16))

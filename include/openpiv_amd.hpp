@@ -2,11 +2,14 @@
 //
 // Mirrors the interface examples/process uses from openpiv-cxx so the engine
 // drops into that driver:
-//   * openpiv_amd::HipFFT          ~ algos::FFT (algos/fft.h:28-295): the
-//     constructor checks power-of-two sizes with the same message
-//     (fft.h:82-83); cross_correlate / cross_correlate_real / auto_correlate
-//     take and return images of the caller's type (fft.h:188-242), so the
-//     correlator_t registry of main.cpp:153-186 can hold it unchanged.
+//   * openpiv_amd::HipFFT          ~ algos::FFT (algos/fft.h:28-295): built
+//     from a size (any type with width() / height(), as core::size) on the
+//     process-wide context, or on an explicit Context; the constructor checks
+//     power-of-two sizes with the same message (fft.h:82-83);
+//     cross_correlate / cross_correlate_real / auto_correlate take and return
+//     images of the caller's type (fft.h:188-242).  openpiv_amd_core.hpp
+//     wraps it as openpiv::algos::HipFFT on openpiv-cxx's own core::image /
+//     core::cf_image types, for the correlator registry of main.cpp:153-186.
 //   * openpiv_amd::Plan            ~ the grid (main.cpp:137-138) plus the whole
 //     per-window loop (main.cpp:189-302) as one batched call per frame pair.
 // Errors surface as std::runtime_error subclasses carrying the C status, as
@@ -19,6 +22,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <complex>
 #include <mutex>
 #include <type_traits>
@@ -61,20 +65,51 @@ private:
     mutable std::mutex m_;
 };
 
+/// The process-wide context: device $OPIV_DEVICE (else 0), created on first
+/// use and kept until the process ends.  Correlators made without an
+/// explicit context (HipFFT(size), as `static algos::FFT fft{ia}` is made at
+/// examples/process/main.cpp:158) share it.
+inline Context& default_context()
+{
+    static Context ctx([] {
+        const char* e = std::getenv("OPIV_DEVICE");
+        return e && *e ? std::atoi(e) : 0;
+    }());
+    return ctx;
+}
+
 /// Square-window FFT correlator with the call surface of algos::FFT.
 /// Thread-safe like the reference class: calls from several threads are
 /// serialised on the context.
 class HipFFT {
 public:
+    /// algos::FFT(const core::size&) (fft.h:76-84) for any size type with
+    /// width() and height(), on the process-wide context
+    /// (the size is checked before the context is opened, so a bad size
+    /// reports the reference's error on any machine)
+    template <class SizeT, class = std::void_t<decltype(std::declval<const SizeT&>().width()),
+                                               decltype(std::declval<const SizeT&>().height())>>
+    explicit HipFFT(const SizeT& size)
+        : HipFFT((check_size(static_cast<uint32_t>(size.width()), static_cast<uint32_t>(size.height())),
+                  default_context()),
+                 static_cast<uint32_t>(size.width()), static_cast<uint32_t>(size.height()))
+    {}
+
     HipFFT(Context& ctx, uint32_t width, uint32_t height) : ctx_(ctx), w_(width), h_(height)
     {
+        check_size(w_, h_);
+    }
+
+    /// fft.h:82-83, and the engine's square windows
+    static void check_size(uint32_t w, uint32_t h)
+    {
         const auto pow2 = [](uint32_t v) { return v && (v & (v - 1)) == 0; };
-        if (!(pow2(w_) && pow2(h_)))
-            throw error(OPIV_E_NOT_POW2, "dimensions must be power of 2: [" + std::to_string(w_) + "," +
-                                             std::to_string(h_) + "]");
-        if (w_ != h_)
-            throw error(OPIV_E_UNSUPPORTED, "only square windows are supported: [" + std::to_string(w_) + "," +
-                                                std::to_string(h_) + "]");
+        if (!(pow2(w) && pow2(h)))
+            throw error(OPIV_E_NOT_POW2, "dimensions must be power of 2: [" + std::to_string(w) + "," +
+                                             std::to_string(h) + "]");
+        if (w != h)
+            throw error(OPIV_E_UNSUPPORTED, "only square windows are supported: [" + std::to_string(w) + "," +
+                                                std::to_string(h) + "]");
     }
 
     /// fft.h:188-206: real(IFFT(FFT(b) conj(FFT(a)))), quadrant-swapped
@@ -123,6 +158,24 @@ public:
         std::lock_guard<std::mutex> lock(ctx_.mutex());
         check(opiv_correlate_windows(ctx_.get(), w_, n, variant, a, b, planes));
     }
+
+    /// transform of n stacked windows of interleaved complex doubles
+    /// (host or device memory; in == out allowed)
+    void transform_raw(const double* in, double* out, opiv_direction d, size_t n = 1) const
+    {
+        std::lock_guard<std::mutex> lock(ctx_.mutex());
+        check(opiv_fft_transform_windows(ctx_.get(), w_, n, d, in, out));
+    }
+
+    /// auto-correlation planes of n stacked real windows
+    void auto_correlate_raw(const double* a, double* planes, size_t n = 1) const
+    {
+        std::lock_guard<std::mutex> lock(ctx_.mutex());
+        check(opiv_auto_correlate_windows(ctx_.get(), w_, n, a, planes));
+    }
+
+    uint32_t width() const { return w_; }
+    uint32_t height() const { return h_; }
 
 private:
     // a real pixel (anything convertible to double, e.g. core::g<T>), a

@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -174,6 +175,11 @@ def _check(rc: int) -> None:
         raise OpivError(rc, msg)
 
 
+def last_error() -> str:
+    """opiv_last_error() of this thread."""
+    return load().opiv_last_error().decode()
+
+
 def image_info(path) -> tuple[int, int, np.dtype, int]:
     """(width, height, pixel dtype, images in the file) of a PNM/TIFF frame."""
     info = ImageInfo()
@@ -238,9 +244,17 @@ class Context:
         _check(self._lib.opiv_ctx_create(device, ctypes.byref(h)))
         self.handle = h
         self.device = device
+        # plans and batches made on this context: closed before it, since
+        # the C objects refer to the context
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, child) -> None:
+        self._children.add(child)
 
     def close(self) -> None:
         if self.handle:
+            for child in list(self._children):
+                child.close()
             self._lib.opiv_ctx_destroy(self.handle)
             self.handle = None
 
@@ -347,6 +361,7 @@ class Plan:
         h = ctypes.c_void_p()
         _check(self._lib.opiv_plan_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)))
         self.handle = h
+        ctx._adopt(self)
         n = ctypes.c_size_t()
         _check(self._lib.opiv_plan_window_count(h, ctypes.byref(n)))
         self.count = n.value
@@ -450,6 +465,7 @@ class Batch:
         h = ctypes.c_void_p()
         _check(self._lib.opiv_batch_create(ctx.handle, ctypes.byref(d), arr, self.npairs, ctypes.byref(h)))
         self.handle = h
+        ctx._adopt(self)
         n = ctypes.c_size_t()
         _check(self._lib.opiv_batch_window_count(h, ctypes.byref(n)))
         self.count = n.value
@@ -614,3 +630,21 @@ def synth_pair(w: int, h: int, *, seed: int = 1, density: float = 0.05, diameter
     if _load_synth().opiv_synth_pair(ctypes.byref(d), a.ctypes.data, b.ctypes.data) != 0:
         raise ValueError("opiv_synth_pair rejected the descriptor")
     return a, b
+
+
+# ---------------------------------------------------------------- workloads
+
+MULTIPASS_SIZES = [(512, 512), (1024, 1024), (2048, 2048), (4096, 4096), (1280, 800)]
+
+
+def multipass_pairs(npairs: int = 24, seed: int = 24, dtype=np.float32):
+    """The uneven multi-pair workload: npairs frame pairs whose sizes are
+    drawn (seeded) from MULTIPASS_SIZES, each with a Lamb-Oseen vortex of core
+    radius 40 px and peak 5 px at a seeded centre.  Yields (a, b) numpy
+    frames one pair at a time."""
+    rng = np.random.default_rng(seed)
+    picks = rng.integers(0, len(MULTIPASS_SIZES), npairs)
+    for i, k in enumerate(picks):
+        w, h = MULTIPASS_SIZES[k]
+        cx, cy = rng.uniform(0.25, 0.75) * w, rng.uniform(0.25, 0.75) * h
+        yield synth_pair(w, h, seed=1000 + i, field=FIELD_LAMB_OSEEN, params=(cx, cy, 40.0, 5.0), dtype=dtype)

@@ -2,10 +2,12 @@
 with the reference's accessors, and the front-end's argument and loader
 errors behave like examples/process (main.cpp:85-132)."""
 import subprocess
+from pathlib import Path
 
 import numpy as np
 import pytest
 
+from conftest import ROOT
 from dropin_util import PROCESS_BIN, build_dropin_check, write_pgm
 
 
@@ -39,3 +41,20 @@ def test_process_rejects_size_mismatch_and_bad_type(tmp_path):
     assert r.returncode == 1 and "unknown fft type" in r.stderr
     r = run([PROCESS_BIN, "--bogus", tmp_path / "a.pgm", tmp_path / "b.pgm"])
     assert r.returncode == 1 and "error parsing options" in r.stderr
+
+
+REF_DIR = Path("/root/reference")
+DROPIN_CORE = ROOT / "oracle" / "_ref" / "dropin_core"
+
+
+@pytest.mark.skipif(not (REF_DIR / "openpiv" / "core" / "image.h").exists(),
+                    reason="openpiv-cxx headers are not mounted here")
+def test_core_types_dropin_compiles_and_checks_sizes():
+    """include/openpiv_amd_core.hpp compiled against the reference's own
+    core::size / core::gf_image / core::cf_image, with algos::HipFFT in the
+    correlator registry of main.cpp:153-186 beside algos::FFT; a
+    non-power-of-two size raises the reference's message before any device
+    is opened"""
+    subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "_ref/dropin_core"], check=True, timeout=600)
+    r = run([DROPIN_CORE, "check"])
+    assert r.returncode == 0 and r.stdout.strip() == "check ok", r.stdout + r.stderr

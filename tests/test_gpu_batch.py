@@ -133,3 +133,35 @@ def test_batch_errors(gpu_ctx, batch_setup):
         oa.Batch(gpu_ctx, pairs, (24,))
     with pytest.raises(oa.OpivError, match="bigger than image"):
         oa.Batch(gpu_ctx, [(pairs[0][0], pairs[0][1], 40, 40)], (64,))
+
+
+def test_multipass_workload_full_size(gpu_ctx):
+    """The benchmark's batch exactly: 24 seeded pairs of 512^2 .. 4096^2 and
+    1280 x 800 with Lamb-Oseen vortices (r_c 40 px, peak 5 px), passes
+    64 / 32 / 16 at 50% overlap in one graph launch.  Every window of every
+    pair and pass is bit-identical to the single-pair launch; the largest
+    pair and a 1280 x 800 pair of each pass are checked against the FP64
+    oracle window by window."""
+    frames = list(oa.multipass_pairs(24, seed=24))
+    sizes = [a.shape for a, _ in frames]
+    say("multipass 24 pairs", sizes)
+    bufs = [(oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)) for a, b in frames]
+    pairs = [(da.ptr, db.ptr, a.shape[1], a.shape[0]) for (a, _), (da, db) in zip(frames, bufs)]
+    batch = oa.Batch(gpu_ctx, pairs, WINS)
+    _, res = run_batch(batch)
+    say("batch windows", batch.count, "shapes", [batch.shape(p) for p in range(3)])
+    big = max(range(len(frames)), key=lambda i: frames[i][0].size)
+    wide = [i for i, s in enumerate(sizes) if s == (800, 1280)]
+    for p, win in enumerate(WINS):
+        for i, (a, _) in enumerate(frames):
+            h, w = a.shape
+            plan = oa.Plan(gpu_ctx, w, h, win, overlap=0.5, dtype=np.float32)
+            one = plan.run(bufs[i][0].ptr, bufs[i][1].ptr)
+            assert batch.block(res, p, i).tobytes() == one.tobytes(), (win, i)
+        say(f"pass {win}: all {len(frames)} pairs bit-identical")
+        for i in [big] + wide[:1]:
+            a, b = frames[i]
+            ref = orc.process_pair(a, b, win, win // 2, win // 2, threads=16)
+            compare(batch.block(res, p, i), ref, f"multipass-w{win}-pair{i}-{a.shape[1]}x{a.shape[0]}",
+                    frames=(a, b, win, win // 2, win // 2))
+            say(f"pass {win} pair {i} {a.shape[1]}x{a.shape[0]} vs oracle ok")

@@ -96,3 +96,14 @@ def test_direct_rejects_other_sizes(gpu_ctx):
     with pytest.raises(oa.OpivError) as e:
         oa.Plan(gpu_ctx, 256, 256, 32, variant=D)
     assert e.value.code == 6
+
+
+def test_direct_uint16_4096_true_75pct_overlap(gpu_ctx):
+    """4096^2 uint16 at 16^2 windows, step 4 (true 75% overlap, 1,042,441
+    windows) on the direct path, every window against the oracle"""
+    say("direct uint16 4096 w16 step 4")
+    a, b = oa.synth_pair(4096, 4096, seed=4, i0=(1000.0, 60000.0), noise=50.0, params=(0.4, 1.1),
+                         dtype=np.uint16)
+    plan = oa.Plan(gpu_ctx, 4096, 4096, 16, step=(4, 4), dtype=np.uint16, variant=D)
+    assert plan.count == 1042441
+    compare(plan.run(a, b), orc.process_pair(a, b, 16, 4, 4, threads=16), "direct-u16-4096-s4")

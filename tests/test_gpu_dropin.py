@@ -178,6 +178,24 @@ def test_process_reads_tiff_like_pgm(tmp_path):
     assert outs[0] == outs[1] and len(outs[0]) > 0
 
 
+def test_process_on_reference_example_tiffs():
+    """process on the reference's own example pair, examples/process/
+    exp1_001_a.tiff (= tests/corr_a.tiff) and exp1_001_b.tiff: RGB,
+    Deflate-compressed, read as grey by the reference's integer rule; the
+    output against the FP64 oracle on the same grey frames"""
+    from conftest import GOLDEN
+    fa, fb = GOLDEN / "corr_a.tiff", GOLDEN / "exp1_001_b.tiff"
+    a, b = oa.read_image(fa), oa.read_image(fb)
+    assert a.shape == b.shape == (369, 511) and a.dtype == np.uint8
+    for size in (16, 32, 64):
+        say(f"process example tiffs -s {size}")
+        r = subprocess.run([str(PROCESS_BIN), "-s", str(size), str(fa), str(fb)], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        ref = orc.process_pair(a, b, size, size // 2, size // 2, threads=8)
+        check_output(r.stdout, [ref[k] for k in ("x", "y", "u", "v", "sn")], (ref["flags"] & 1) != 0)
+
+
 def test_read_image_into_device_memory(tmp_path, gpu_ctx):
     rng = np.random.default_rng(5)
     a = rng.integers(0, 65536, (90, 70)).astype(np.uint16)
@@ -187,3 +205,31 @@ def test_read_image_into_device_memory(tmp_path, gpu_ctx):
     gpu_ctx.read_image_device(tmp_path / "a.tif", buf.ptr, pitch=pitch, index=1)
     got = buf.download(np.zeros((90, pitch), np.uint16))[:, :70]
     assert np.array_equal(got, a[::-1])
+
+
+DROPIN_CORE = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "dropin_core"
+
+
+@pytest.mark.skipif(not DROPIN_CORE.exists(), reason="oracle/_ref/dropin_core is built where openpiv-cxx is mounted")
+@pytest.mark.parametrize("win", [16, 32, 64])
+def test_core_types_registry_dropin(tmp_path, win):
+    """algos::HipFFT (include/openpiv_amd_core.hpp) in examples/process's
+    correlator registry on the reference's own core::gf_image frames, with
+    the reference's extract / find_peaks / fit_simple_gaussian and ThreadPool,
+    against algos::FFT in the same binary: integer peaks equal, vectors within
+    1e-3 px (complex and real variants); transform and auto_correlate within
+    1e-5 of the largest magnitude; a size mismatch raises the reference's text"""
+    say(f"dropin_core win {win}")
+    a, b = oa.synth_pair(256, 192, seed=31 + win, params=(2.3, -1.7), dtype=np.float64)
+    a.tofile(tmp_path / "a.bin")
+    b.tofile(tmp_path / "b.bin")
+    r = subprocess.run([str(DROPIN_CORE), "run", str(tmp_path / "a.bin"), str(tmp_path / "b.bin"), "256", "192",
+                        str(win), str(win // 2), "4"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    f = r.stdout.split()
+    vals = {f[i]: float(f[i + 1]) for i in range(0, len(f) - 1, 2)}
+    say(r.stdout.strip())
+    assert vals["windows"] > 0 and vals["mismatches"] == 0
+    assert vals["max_err"] <= 1e-3 and vals["real_err"] <= 1e-3
+    assert vals["transform_err"] <= 1e-5 and vals["auto_err"] <= 1e-5
+    assert vals["size_error"] == 1

@@ -29,14 +29,43 @@ def say(*a):
     print(*a, flush=True)
 
 
-def compare(res, ref, label, max_tie=MAX_TIE_FRACTION, max_peak1_diff=0):
+# the engine's NEAR_TIE tolerance: FP32 correlation values are within this
+# share of peak0 of the FP64 ones
+FP32_REACH = 2e-6
+
+
+def second_peak_tie(frames, i, p0, p1s):
+    """Whether window i's FP64 plane holds an element, at least as high as
+    the lower of the two second peaks p1s, whose strict-maximum test (the
+    4-neighbour comparison of find_peaks, image_utils.impl.h:48-57) is
+    decided by less than FP32 reach: then FP32 and FP64 may keep different
+    second peaks (s/n only; the first peak and the vector are unaffected).
+    frames = (a, b, win, step_x, step_y)."""
+    a, b, win, sx, sy = frames
+    h, w = a.shape
+    nx, ny = 1 + (w - win) // sx, 1 + (h - win) // sy
+    x0 = (w - win - sx * (nx - 1)) // 2 + (i % nx) * sx
+    y0 = (h - win - sy * (ny - 1)) // 2 + (i // nx) * sy
+    pl = orc.FFT(win).cross_correlate(a[y0:y0 + win, x0:x0 + win].astype(np.float64),
+                                      b[y0:y0 + win, x0:x0 + win].astype(np.float64))
+    c = pl[1:win - 2, 1:win - 1]
+    nb = np.maximum.reduce([pl[0:win - 3, 1:win - 1], pl[2:win - 1, 1:win - 1],
+                            pl[1:win - 2, 0:win - 2], pl[1:win - 2, 2:win]])
+    reach = FP32_REACH * abs(p0)
+    return bool(((c >= min(p1s) - reach) & (np.abs(c - nb) <= reach)).any())
+
+
+def compare(res, ref, label, max_tie=MAX_TIE_FRACTION, max_peak1_diff=0, frames=None):
     """res: engine results (RESULT_DTYPE); ref: oracle VECTOR_DTYPE or golden dict.
     Windows the engine flags as near/exact ties are exempt from the peak
     comparisons (the target's counted exception); their share is bounded.
     max_peak1_diff: windows whose second peak value (and so s/n) may differ
     because another element's strict-maximum test is an exact integer tie
     that FP64 and FP32 rounding break differently; their peak positions,
-    two-peak decisions and vectors are still compared."""
+    two-peak decisions and vectors are still compared.  With frames =
+    (a, b, win, step_x, step_y), a window whose second peak differs is also
+    accepted when its FP64 plane shows such a strict-maximum test within FP32
+    reach (second_peak_tie); it is counted and reported."""
     get = (lambda k: ref[k]) if isinstance(ref, np.ndarray) else (lambda k: ref[f"{label}_{k}"])
     assert len(res) == len(get("u"))
     few_ref = (get("flags") & 1) != 0
@@ -64,7 +93,14 @@ def compare(res, ref, label, max_tie=MAX_TIE_FRACTION, max_peak1_diff=0):
     rel1 = np.abs(res["peak1"][fin] - p1) / np.abs(p0)
     assert rel0.max(initial=0) <= PLANE_RTOL, (label, rel0.max(), np.flatnonzero(fin)[rel0.argmax()])
     p1_diff = rel1 > PLANE_RTOL
-    assert p1_diff.sum() <= max_peak1_diff, (label, rel1.max(), np.flatnonzero(fin)[p1_diff][:8])
+    unexplained = p1_diff.copy()
+    if frames is not None and p1_diff.sum() > max_peak1_diff:
+        idx = np.flatnonzero(fin)
+        for k in np.flatnonzero(p1_diff):
+            i = idx[k]
+            if second_peak_tie(frames, i, get("peak0")[i], (res["peak1"][i], get("peak1")[i])):
+                unexplained[k] = False
+    assert unexplained.sum() <= max_peak1_diff, (label, rel1.max(), np.flatnonzero(fin)[unexplained][:8])
     sn_ref = get("sn")[fin]
     sn_tol = SN_RTOL * (1.0 + np.abs(p0) / np.maximum(np.abs(p1), 1e-300)) * np.abs(sn_ref)
     assert (np.abs(res["sn"][fin] - sn_ref) <= sn_tol)[~p1_diff].all(), label
