@@ -230,6 +230,22 @@ int opiv_vjp_pair(opiv_plan* plan, const void* a, const void* b, const opiv_wind
 int opiv_vjp_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, const opiv_window_result* d_fwd,
                         const double* d_cot, float* d_grad_a, float* d_grad_b);
 
+/* Allocation-free form of opiv_vjp_pair_async.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results; the per-window tap weights go to a
+ * fixed scratch array of OPIV_NOALLOC_MAX_WINDOWS windows (32 B each, 8 MiB),
+ * one per GPU, which each GPU the library runs on holds until the process
+ * ends, instead of a buffer the context grows.  Calls no hipMalloc or
+ * hipFree.  Plans with more windows get OPIV_E_UNSUPPORTED.  Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (from
+ * different contexts or streams) race; calls on one context's stream are
+ * ordered and safe.  The array keeps the last call's tap weights; every call
+ * writes all of its windows' weights before reading them. */
+#define OPIV_NOALLOC_MAX_WINDOWS 262144
+int opiv_vjp_pair_async_noalloc(opiv_plan* plan, const void* d_a, const void* d_b, const opiv_window_result* d_fwd,
+                                const double* d_cot, float* d_grad_a, float* d_grad_b);
+
 /* var[2i], var[2i+1] = sigma^2 * |d(u_i, v_i)/d(both frames)|^2: the
  * variance of (u_i, v_i) under i.i.d. pixel noise sigma (linear propagation) */
 int opiv_variance_pair(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,35 @@
 #include "transform.hpp"
 
 using namespace opiv::detail;
+
+namespace {
+
+// Scratch of opiv_vjp_pair_async_noalloc: the VJP's per-window tap weights
+// (32 B each) for callers that cannot allocate.  One copy per GPU, held until
+// the process ends; every context on a GPU shares it.  Host code never uses
+// this name as an address: the device address of each GPU's copy is looked
+// up once per device (scratch_address) and kept.
+static __device__ __align__(256) opiv::AdjCoef scratch[OPIV_NOALLOC_MAX_WINDOWS];
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;      // device -> address of its copy of scratch
+
+// the address of the current device's copy (resolved on the first call per
+// device, with that device current)
+int scratch_address(int device, void** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[(size_t)device]) {
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[(size_t)device] = ptr;
+    }
+    *out = g_scratch_ptr[(size_t)device];
+    return OPIV_OK;
+}
+
+} // namespace
 
 extern "C" {
 
@@ -44,8 +74,11 @@ int opiv_ctx_create(int device, opiv_ctx** out)
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(OPIV_E_NO_DEVICE, "device %d is %s; this library is built for gfx950 only", device, prop.gcnArchName);
     HIP_TRY(hipSetDevice(device));
+    void* scratch_ptr = nullptr;
+    if (int rc = scratch_address(device, &scratch_ptr)) return rc;
     opiv_ctx* c = new opiv_ctx;
     c->device = device;
+    c->scratch_coef = scratch_ptr;
     c->cu_count = prop.multiProcessorCount;
     c->arch = prop.gcnArchName;
     c->arch = c->arch.substr(0, c->arch.find(':'));   // "gfx950:sramecc+:xnack-" -> "gfx950"
@@ -333,13 +366,17 @@ static int adjoint_setup(opiv_plan* plan, const void* a, const void* b, const op
     return OPIV_OK;
 }
 
-// the VJP's second half: tap weights and gradients into device buffers, on
-// the context stream
-static int launch_vjp(opiv_plan* plan, opiv::AdjParams& p, const double* d_cot, float* d_ga, float* d_gb)
+// the VJP's second half: tap weights (into coef, or the context's buffer
+// when coef is null) and gradients into device buffers, on the context stream
+static int launch_vjp(opiv_plan* plan, opiv::AdjParams& p, const double* d_cot, float* d_ga, float* d_gb,
+                      opiv::AdjCoef* coef = nullptr)
 {
     opiv_ctx* ctx = plan->ctx;
-    HIP_TRY(ctx->adj_coef.ensure((size_t)p.n_windows * sizeof(opiv::AdjCoef)));
-    p.coef = (opiv::AdjCoef*)ctx->adj_coef.p;
+    if (!coef) {
+        HIP_TRY(ctx->adj_coef.ensure((size_t)p.n_windows * sizeof(opiv::AdjCoef)));
+        coef = (opiv::AdjCoef*)ctx->adj_coef.p;
+    }
+    p.coef = coef;
     p.cot = d_cot;
     p.grad_a = d_ga;
     p.grad_b = d_gb;
@@ -363,6 +400,31 @@ int opiv_vjp_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, const
     HIP_TRY(hipSetDevice(plan->ctx->device));
     opiv::AdjParams p = adjoint_params(plan, d_a, d_b, d_fwd);
     return launch_vjp(plan, p, d_cot, d_grad_a, d_grad_b);
+}
+
+int opiv_vjp_pair_async_noalloc(opiv_plan* plan, const void* d_a, const void* d_b, const opiv_window_result* d_fwd,
+                                const double* d_cot, float* d_grad_a, float* d_grad_b)
+{
+    if (!plan || !d_a || !d_b || !d_fwd || !d_cot || !d_grad_a || !d_grad_b)
+        return fail(OPIV_E_INVALID, "opiv_vjp_pair_async_noalloc: null argument");
+    const size_t count = (size_t)plan->nx * plan->ny;
+    if (count > OPIV_NOALLOC_MAX_WINDOWS)
+        return fail(OPIV_E_UNSUPPORTED, "opiv_vjp_pair_async_noalloc: %zu windows exceed the scratch array's %d",
+                    count, OPIV_NOALLOC_MAX_WINDOWS);
+    if (plan->desc.variant == OPIV_REAL_COMPAT)
+        return fail(OPIV_E_UNSUPPORTED, "the adjoint supports the circular correlation only");
+    if (int rc = check_frames(plan, d_a, d_b, "opiv_vjp_pair_async_noalloc")) return rc;
+    for (const void* q : { d_a, d_b, (const void*)d_fwd, (const void*)d_cot, (const void*)d_grad_a,
+                           (const void*)d_grad_b })
+        if (!is_device_ptr(q))
+            return fail(OPIV_E_INVALID, "opiv_vjp_pair_async_noalloc: every buffer must be device memory");
+    opiv_ctx* ctx = plan->ctx;
+    if (!ctx->scratch_coef)
+        return fail(OPIV_E_HIP, "opiv_vjp_pair_async_noalloc: the scratch array of device %d is not mapped",
+                    ctx->device);
+    HIP_TRY(hipSetDevice(ctx->device));
+    opiv::AdjParams p = adjoint_params(plan, d_a, d_b, d_fwd);
+    return launch_vjp(plan, p, d_cot, d_grad_a, d_grad_b, static_cast<opiv::AdjCoef*>(ctx->scratch_coef));
 }
 
 int opiv_vjp_pair(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,

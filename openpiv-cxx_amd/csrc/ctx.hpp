@@ -69,6 +69,7 @@ struct opiv_ctx {
     opiv::detail::DevBuf stage_a, stage_b, stage_out;
     opiv::detail::DevBuf stage_fwd, stage_cot, stage_ga, stage_gb, stage_var;
     opiv::detail::DevBuf adj_coef;   // VJP tap weights, 32 B per window
+    void* scratch_coef = nullptr;    // this device's copy of the allocation-free VJP's scratch array
     // background subtraction: staged host frames and outputs, mean, pointer
     // tables, per-frame range keys
     opiv::detail::DevBuf bg_frames, bg_out, bg_mean, bg_table, bg_range;

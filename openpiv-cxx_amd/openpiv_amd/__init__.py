@@ -59,7 +59,7 @@ EXPORTS = (
     "opiv_correlate_windows", "opiv_auto_correlate_windows", "opiv_fft_transform_windows",
     "opiv_batch_create", "opiv_batch_destroy", "opiv_batch_window_count", "opiv_batch_offsets",
     "opiv_batch_set_pairs", "opiv_batch_shape", "opiv_batch_run",
-    "opiv_vjp_pair", "opiv_vjp_pair_async", "opiv_variance_pair",
+    "opiv_vjp_pair", "opiv_vjp_pair_async", "opiv_vjp_pair_async_noalloc", "opiv_variance_pair",
     "opiv_average_subtract",
     "opiv_image_probe", "opiv_image_read",
 )
@@ -158,6 +158,7 @@ def load() -> ctypes.CDLL:
     lib.opiv_batch_run.argtypes = [vp, vp, sz]
     lib.opiv_vjp_pair.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.opiv_vjp_pair_async.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.opiv_vjp_pair_async_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.opiv_variance_pair.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp]
     lib.opiv_average_subtract.argtypes = [vp, vp, sz, u32, u32, sz, i32, vp, vp]
     lib.opiv_image_probe.argtypes = [ctypes.c_char_p, ctypes.POINTER(ImageInfo)]
