@@ -168,6 +168,10 @@ int upload_table(opiv_batch* b)
 int tune(opiv_batch* b, uint32_t pass, opiv_window_result* scratch)
 {
     const int cu = b->ctx->cu_count > 0 ? b->ctx->cu_count : 256;
+    if (b->desc.win[pass] >= 128) {         // one 256-thread workgroup per window: nothing to choose
+        b->shape[pass] = Shape{256, 0};
+        return OPIV_OK;
+    }
     const std::vector<Shape> cands = {{256, 0}, {64, 0}, {256, 2 * cu}, {256, 4 * cu}, {64, 8 * cu}, {64, 16 * cu}};
     const opiv::CorrParams p = pass_params(b, pass, scratch);
     hipEvent_t e0, e1;

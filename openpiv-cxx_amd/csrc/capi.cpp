@@ -339,12 +339,22 @@ static opiv::AdjParams adjoint_params(const opiv_plan* plan, const void* da, con
     return p;
 }
 
+// the adjoint kernels take windows up to 64 x 64
+static int adjoint_window_check(const opiv_plan* plan)
+{
+    if (plan->desc.win > 64)
+        return fail(OPIV_E_UNSUPPORTED, "the adjoint takes windows up to 64x64, not %ux%u", plan->desc.win,
+                    plan->desc.win);
+    return OPIV_OK;
+}
+
 static int adjoint_setup(opiv_plan* plan, const void* a, const void* b, const opiv_window_result* fwd,
                          opiv::AdjParams& p)
 {
     opiv_ctx* ctx = plan->ctx;
     if (plan->desc.variant == OPIV_REAL_COMPAT)
         return fail(OPIV_E_UNSUPPORTED, "the adjoint supports the circular correlation only");
+    if (int rc = adjoint_window_check(plan)) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t count = (size_t)plan->nx * plan->ny;
     const size_t frame_bytes = plan->desc.pitch * plan->desc.img_h * dtype_size(plan->desc.dtype);
@@ -392,6 +402,7 @@ int opiv_vjp_pair_async(opiv_plan* plan, const void* d_a, const void* d_b, const
         return fail(OPIV_E_INVALID, "opiv_vjp_pair_async: null argument");
     if (plan->desc.variant == OPIV_REAL_COMPAT)
         return fail(OPIV_E_UNSUPPORTED, "the adjoint supports the circular correlation only");
+    if (int rc = adjoint_window_check(plan)) return rc;
     if (int rc = check_frames(plan, d_a, d_b, "opiv_vjp_pair_async")) return rc;
     for (const void* q : { d_a, d_b, (const void*)d_fwd, (const void*)d_cot, (const void*)d_grad_a,
                            (const void*)d_grad_b })
@@ -413,6 +424,7 @@ int opiv_vjp_pair_async_noalloc(opiv_plan* plan, const void* d_a, const void* d_
                     count, OPIV_NOALLOC_MAX_WINDOWS);
     if (plan->desc.variant == OPIV_REAL_COMPAT)
         return fail(OPIV_E_UNSUPPORTED, "the adjoint supports the circular correlation only");
+    if (int rc = adjoint_window_check(plan)) return rc;
     if (int rc = check_frames(plan, d_a, d_b, "opiv_vjp_pair_async_noalloc")) return rc;
     for (const void* q : { d_a, d_b, (const void*)d_fwd, (const void*)d_cot, (const void*)d_grad_a,
                            (const void*)d_grad_b })
@@ -572,6 +584,7 @@ int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t di
 {
     if (!ctx || !in || !out) return fail(OPIV_E_INVALID, "transform: null argument");
     if (int rc = check_window(win)) return rc;
+    if (win > 64) return fail(OPIV_E_UNSUPPORTED, "the transform kernel takes windows up to 64x64, not %ux%u", win, win);
     if (direction != OPIV_FORWARD && direction != OPIV_REVERSE)
         return fail(OPIV_E_INVALID, "unknown direction %d", direction);
     if (n == 0) return OPIV_OK;

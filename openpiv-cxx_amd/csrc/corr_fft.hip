@@ -481,6 +481,8 @@ static hipError_t launch_dt(const CorrParams& p, int win, bool planes, int nt, i
     case 64:
         if constexpr (DT == 2) return launch_w64<DT>(p, planes, nt, blocks, stream);
         else return launch_w<64, DT>(p, planes, nt, blocks, stream);
+    case 128:   // one workgroup of 256 threads per window, never persistent
+        return nt == 256 ? launch_corr_fft128(p, DT, planes, stream) : hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 }

@@ -42,4 +42,8 @@ hipError_t launch_corr_fft(const CorrParams& p, int win, int dtype, bool planes,
 hipError_t launch_corr_fft_shape(const CorrParams& p, int win, int dtype, bool planes, int nt, int blocks,
                                  hipStream_t stream);
 
+// 128 x 128 windows (corr_fft128.hip): one workgroup of 256 threads per
+// window, single pair or batch (p.pairs), fused or plane mode
+hipError_t launch_corr_fft128(const CorrParams& p, int dtype, bool planes, hipStream_t stream);
+
 } // namespace opiv

@@ -11,7 +11,7 @@ namespace opiv {
 
 // square window sides the engine correlates (powers of two, fft.h:76-84)
 constexpr uint32_t kMinWin = 8;
-constexpr uint32_t kMaxWin = 64;
+constexpr uint32_t kMaxWin = 128;
 
 // window sizes the direct (MFMA) path implements
 inline bool direct_supported(int win) { return win == 16; }
@@ -37,8 +37,11 @@ struct BatchPair {
 
 // windows per wave task of the kernel for window size win and pixel type
 // dtype (0 u8, 1 u16, 2 f32, 3 f64): two rounds of 64 / win lane groups, one
-// round for f64 frames at 64 x 64
-constexpr int batch_align(int win, int dtype) { return (win == 64 && dtype == 3 ? 1 : 2) * (64 / win); }
+// round for f64 frames at 64 x 64; one window per workgroup at 128 x 128
+constexpr int batch_align(int win, int dtype)
+{
+    return win >= 128 ? 1 : (win == 64 && dtype == 3 ? 1 : 2) * (64 / win);
+}
 
 // Largest frame row pitch in bytes.  The FFT kernel addresses a wave task's
 // pixels as 32-bit byte offsets from the task's first window: a task spans

@@ -165,3 +165,20 @@ def test_multipass_workload_full_size(gpu_ctx):
             compare(batch.block(res, p, i), ref, f"multipass-w{win}-pair{i}-{a.shape[1]}x{a.shape[0]}",
                     frames=(a, b, win, win // 2, win // 2))
             say(f"pass {win} pair {i} {a.shape[1]}x{a.shape[0]} vs oracle ok")
+
+
+def test_batch_with_128_pass(gpu_ctx, batch_setup):
+    """a 128 x 128 pass (one workgroup per window) in a batch beside 32 x 32:
+    bit-identical to the single-pair launches; pairs smaller than the window
+    are refused by the grid as in the reference"""
+    frames, bufs, pairs = batch_setup
+    keep = [i for i, (a, _) in enumerate(frames) if min(a.shape) >= 128]
+    sub = [pairs[i] for i in keep]
+    batch = oa.Batch(gpu_ctx, sub, (128, 32))
+    _, res = run_batch(batch)
+    assert batch.shape(0) == (256, 0)
+    for p, win in enumerate((128, 32)):
+        for j, i in enumerate(keep):
+            a, b = frames[i]
+            plan = oa.Plan(gpu_ctx, a.shape[1], a.shape[0], win, overlap=0.5, dtype=a.dtype)
+            assert batch.block(res, p, j).tobytes() == plan.run(a, b).tobytes(), (win, i)

@@ -116,7 +116,7 @@ def compare(res, ref, label, max_tie=MAX_TIE_FRACTION, max_peak1_diff=0, frames=
 
 # ------------------------------------------------------------------ planes
 
-@pytest.mark.parametrize("win", [8, 16, 32, 64])
+@pytest.mark.parametrize("win", [8, 16, 32, 64, 128])
 @pytest.mark.parametrize("variant", [oa.CIRCULAR_FFT_FP32, oa.REAL_COMPAT])
 def test_correlation_planes(gpu_ctx, win, variant):
     say(f"planes win={win} variant={variant}")
@@ -210,7 +210,7 @@ def rotation_pair(w, dmax=6.0, dtype=np.float32, seed=2):
     return oa.synth_pair(w, w, seed=seed, field=oa.FIELD_ROTATION, params=(dmax, r), dtype=dtype)
 
 
-@pytest.mark.parametrize("win,step", [(32, 16), (64, 32)])
+@pytest.mark.parametrize("win,step", [(32, 16), (64, 32), (128, 64)])
 def test_rotation_4096_f32(gpu_ctx, win, step):
     """4096^2 float32, solid-body rotation |d|max = 6 px: every window
     against the FP64 oracle."""
@@ -243,6 +243,35 @@ def test_limit_search_and_real_against_oracle(gpu_ctx):
         plan = oa.Plan(gpu_ctx, 1024, 768, 32, dtype=np.float32, variant=variant, limit_search=limit)
         ref = orc.process_pair(a, b, 32, 16, 16, variant=variant, limit_search=limit, threads=16)
         compare(plan.run(a, b), ref, f"v{variant}-l{int(limit)}")
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float64])
+def test_window_128_dtypes_limit_and_real(gpu_ctx, dtype):
+    """128 x 128 windows (one workgroup per window): every pixel type, and
+    the limit-search view and REAL_COMPAT variant, against the oracle"""
+    i0 = (30.0, 250.0) if dtype == np.uint8 else (1000.0, 60000.0)
+    a, b = oa.synth_pair(1024, 768, seed=12, i0=i0, noise=3.0 if dtype == np.uint8 else 50.0,
+                         params=(-5.3, 7.6), dtype=dtype)
+    for variant, limit in ((oa.CIRCULAR_FFT_FP32, False), (oa.CIRCULAR_FFT_FP32, True), (oa.REAL_COMPAT, False),
+                           (oa.REAL_COMPAT, True)):
+        say(f"w128 {np.dtype(dtype).name} variant={variant} limit={limit}")
+        plan = oa.Plan(gpu_ctx, 1024, 768, 128, overlap=0.5, dtype=dtype, variant=variant, limit_search=limit)
+        ref = orc.process_pair(a, b, 128, 64, 64, variant=variant, limit_search=limit, threads=16)
+        compare(plan.run(a, b), ref, f"w128-{np.dtype(dtype).name}-v{variant}-l{int(limit)}")
+
+
+def test_window_128_rejected_where_unsupported(gpu_ctx):
+    with pytest.raises(oa.OpivError) as e:
+        gpu_ctx.transform(np.zeros((128, 128)))
+    assert e.value.code == 6
+    plan = oa.Plan(gpu_ctx, 512, 512, 128, dtype=np.float32)
+    a = np.zeros((512, 512), np.float32)
+    with pytest.raises(oa.OpivError) as e:
+        plan.vjp(a, a, np.zeros((plan.count, 2)))
+    assert e.value.code == 6
+    with pytest.raises(oa.OpivError) as e:
+        oa.Plan(gpu_ctx, 512, 512, 128, variant=oa.DIRECT_BF16_SPLIT)
+    assert e.value.code == 6
 
 
 # ------------------------------------------------------------------ edges
@@ -305,7 +334,7 @@ def test_errors(gpu_ctx):
         oa.Plan(gpu_ctx, 256, 256, 24)
     assert e.value.code == 2
     with pytest.raises(oa.OpivError) as e:
-        oa.Plan(gpu_ctx, 256, 256, 128)
+        oa.Plan(gpu_ctx, 512, 512, 256)
     assert e.value.code == 6
     with pytest.raises(oa.OpivError, match="offsets must be between"):
         oa.Plan(gpu_ctx, 256, 256, 32, overlap=1.5)
