@@ -433,7 +433,8 @@ def main():
     ap.add_argument("--step", type=int, default=0, help="grid step (default: half the window, 50%% overlap)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--path", choices=["auto", "fft", "direct"], default="auto",
-                    help="correlation path: fused FP32 FFT, BF16-split MFMA direct (16^2), or the faster")
+                    help="correlation path: fused FP32 FFT, direct MFMA (16^2; 32^2 for 8/16-bit frames), "
+                         "or the faster")
     ap.add_argument("--dtype", choices=["f32", "u16", "u8"], default="f32")
     ap.add_argument("--adjoint", action="store_true",
                     help="f32 differentiation workload: forward vs VJP vs variance (prints its own line)")
@@ -540,13 +541,16 @@ def main():
     # flops per window (5 N log2 N per complex N-point transform, halved for
     # real input: two forward transforms, the half-spectrum product, one
     # inverse) against the FP32 vector peak.  Direct path: the window's GEMM,
-    # 2 W^4 operations per product (one int8 product for 8-bit frames, 4 or 6
-    # split bf16 products otherwise) against the MFMA peak of its type.
+    # 2 W^4 operations per product against the MFMA peak of its type: int8
+    # for 8-bit frames (one product) and for 16-bit frames at 32 x 32 (four
+    # byte-plane products), split bf16 otherwise (4 products for u16, 6 for
+    # float frames).
+    direct_i8 = chosen == "direct" and (np_dtype == np.uint8 or (np_dtype == np.uint16 and win == 32))
     if chosen == "fft":
         per_win = fft_flops(win)
         peak, unit, kind = FP32_PEAK_TFLOPS, "TFLOP/s", "fp32_valu"
-    elif np_dtype == np.uint8:
-        per_win = 2.0 * win ** 4
+    elif direct_i8:
+        per_win = 2.0 * win ** 4 * (1 if np_dtype == np.uint8 else 4)
         peak, unit, kind = I8_MFMA_PEAK_TOPS, "TOP/s", "mfma_i8"
     else:
         per_win = 2.0 * win ** 4 * (4 if np_dtype == np.uint16 else 6)
@@ -563,7 +567,6 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(a, b, win, step)
 
-    direct_i8 = chosen == "direct" and np_dtype == np.uint8
     path_name = {"fft": "fft_fp32_fused",
                  "direct": "direct_i8_mfma" if direct_i8 else "direct_bf16_split_mfma"}[chosen]
     line = {

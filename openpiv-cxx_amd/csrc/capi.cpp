@@ -253,8 +253,8 @@ static int correlate_stacked(opiv_ctx* ctx, uint32_t win, size_t n, int variant,
     if (variant < OPIV_CIRCULAR_FFT_FP32 || variant > OPIV_AUTO)
         return fail(OPIV_E_UNSUPPORTED, "unknown variant %d", variant);
     if (variant == OPIV_AUTO) variant = auto_variant((int)win, OPIV_F64);
-    if (variant == OPIV_DIRECT_BF16_SPLIT && !opiv::direct_supported((int)win))
-        return fail(OPIV_E_UNSUPPORTED, "the direct path does not implement %ux%u windows", win, win);
+    if (variant == OPIV_DIRECT_BF16_SPLIT && !opiv::direct_supported((int)win, OPIV_F64))
+        return fail(OPIV_E_UNSUPPORTED, "the direct path does not implement %ux%u windows of FP64 values", win, win);
     if (n == 0) return OPIV_OK;
     if (n > 0x7fffffffull / win) return fail(OPIV_E_UNSUPPORTED, "too many windows");
     HIP_TRY(hipSetDevice(ctx->device));

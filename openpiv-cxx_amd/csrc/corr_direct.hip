@@ -389,6 +389,7 @@ static hipError_t launch_direct_dt(const CorrParams& p, bool planes, hipStream_t
 hipError_t launch_corr_direct(const CorrParams& p, int win, int dtype, bool planes, hipStream_t stream)
 {
     if (p.n_windows <= 0) return hipSuccess;
+    if (win == 32) return launch_corr_direct32(p, dtype, planes, stream);
     if (win != 16 || p.real_compat) return hipErrorInvalidValue;
     switch (dtype) {
     case 0: return launch_direct_dt<0>(p, planes, stream);

@@ -12,4 +12,7 @@ namespace opiv {
 // window); the circular variant only
 hipError_t launch_corr_direct(const CorrParams& p, int win, int dtype, bool planes, hipStream_t stream);
 
+// 32 x 32 windows of 8- or 16-bit frames (corr_direct32.hip)
+hipError_t launch_corr_direct32(const CorrParams& p, int dtype, bool planes, hipStream_t stream);
+
 } // namespace opiv

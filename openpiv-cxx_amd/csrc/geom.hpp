@@ -13,8 +13,10 @@ namespace opiv {
 constexpr uint32_t kMinWin = 8;
 constexpr uint32_t kMaxWin = 128;
 
-// window sizes the direct (MFMA) path implements
-inline bool direct_supported(int win) { return win == 16; }
+// window sizes and pixel types (0 u8, 1 u16, 2 f32, 3 f64) the direct
+// (MFMA) path implements: 16 x 16 for every pixel type, 32 x 32 for 8- and
+// 16-bit frames (int8 byte planes)
+inline bool direct_supported(int win, int dtype) { return win == 16 || (win == 32 && (dtype == 0 || dtype == 1)); }
 
 // One frame pair of a batch pass (device table): its frames and its grid.
 // The pass's windows are numbered pair by pair, pair k owning

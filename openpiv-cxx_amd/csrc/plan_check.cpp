@@ -66,7 +66,7 @@ int check_window(uint32_t win)
 // f32: 89 against 309 us).
 int32_t auto_variant(int win, int dtype)
 {
-    return direct_supported(win) && dtype == OPIV_U8 ? OPIV_DIRECT_BF16_SPLIT : OPIV_CIRCULAR_FFT_FP32;
+    return win == 16 && dtype == OPIV_U8 ? OPIV_DIRECT_BF16_SPLIT : OPIV_CIRCULAR_FFT_FP32;
 }
 
 int check_plan_desc(const opiv_plan_desc* d, PlanGeom* g)
@@ -78,8 +78,9 @@ int check_plan_desc(const opiv_plan_desc* d, PlanGeom* g)
     if (es == 0) return fail(OPIV_E_INVALID, "unknown dtype %d", d->dtype);
     if (d->variant < OPIV_CIRCULAR_FFT_FP32 || d->variant > OPIV_AUTO)
         return fail(OPIV_E_UNSUPPORTED, "unknown variant %d", d->variant);
-    if (d->variant == OPIV_DIRECT_BF16_SPLIT && !direct_supported((int)win))
-        return fail(OPIV_E_UNSUPPORTED, "the direct path does not implement %ux%u windows", win, win);
+    if (d->variant == OPIV_DIRECT_BF16_SPLIT && !direct_supported((int)win, d->dtype))
+        return fail(OPIV_E_UNSUPPORTED, "the direct path does not implement %ux%u windows of this pixel type", win,
+                    win);
     const size_t pitch = d->pitch ? d->pitch : d->img_w;
     if (pitch < d->img_w) return fail(OPIV_E_INVALID, "pitch %zu smaller than width %u", pitch, d->img_w);
     if (pitch > kMaxPitchBytes / es)

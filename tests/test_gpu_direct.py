@@ -107,3 +107,55 @@ def test_direct_uint16_4096_true_75pct_overlap(gpu_ctx):
     plan = oa.Plan(gpu_ctx, 4096, 4096, 16, step=(4, 4), dtype=np.uint16, variant=D)
     assert plan.count == 1042441
     compare(plan.run(a, b), orc.process_pair(a, b, 16, 4, 4, threads=16), "direct-u16-4096-s4")
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_direct32_dtypes(gpu_ctx, dtype):
+    """32 x 32 windows on the int8 MFMA path (8-bit pixels, or 16-bit pixels
+    as two exact byte planes), ragged frames, against the oracle"""
+    i0 = (30.0, 250.0) if dtype == np.uint8 else (1000.0, 60000.0)
+    a, b = oa.synth_pair(777, 519, seed=23, i0=i0, noise=3.0 if dtype == np.uint8 else 50.0,
+                         params=(0.4, 1.1), dtype=dtype)
+    for step in (16, 8, 32):
+        say(f"direct32 {np.dtype(dtype).name} step {step}")
+        plan = oa.Plan(gpu_ctx, 777, 519, 32, step=(step, step), dtype=dtype, variant=D)
+        compare(plan.run(a, b), orc.process_pair(a, b, 32, step, step, threads=16),
+                f"direct32-{np.dtype(dtype).name}-s{step}")
+
+
+def test_direct32_limit_search_and_pitch(gpu_ctx):
+    a, b = oa.synth_pair(640, 480, seed=29, i0=(30.0, 250.0), noise=3.0, params=(-3.1, 2.2), dtype=np.uint8)
+    plan = oa.Plan(gpu_ctx, 640, 480, 32, dtype=np.uint8, variant=D, limit_search=True)
+    compare(plan.run(a, b), orc.process_pair(a, b, 32, 16, 16, limit_search=True, threads=16), "direct32-limit")
+    # rows of 643 bytes: the element-load path
+    pa = np.zeros((480, 643), np.uint8)
+    pb = np.zeros((480, 643), np.uint8)
+    pa[:, :640], pb[:, :640] = a, b
+    plan = oa.Plan(gpu_ctx, 640, 480, 32, dtype=np.uint8, variant=D, pitch=643)
+    ref = oa.Plan(gpu_ctx, 640, 480, 32, dtype=np.uint8, variant=D)
+    assert plan.run(pa, pb).tobytes() == ref.run(a, b).tobytes()
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_direct32_4096(gpu_ctx, dtype):
+    """4096^2 pairs at 32^2 / step 16 (65025 windows) on the direct path:
+    the uint16 pair of the precision workload (I0 ~ U(1000, 60000), noise 50,
+    shift (0.4, 1.1)) and its 8-bit counterpart, every window against the
+    oracle"""
+    say(f"direct32 4096 {np.dtype(dtype).name}")
+    if dtype == np.uint16:
+        a, b = oa.synth_pair(4096, 4096, seed=4, i0=(1000.0, 60000.0), noise=50.0, params=(0.4, 1.1), dtype=dtype)
+    else:
+        a, b = oa.synth_pair(4096, 4096, seed=4, i0=(30.0, 250.0), noise=3.0, params=(0.4, 1.1), dtype=dtype)
+    plan = oa.Plan(gpu_ctx, 4096, 4096, 32, step=(16, 16), dtype=dtype, variant=D)
+    compare(plan.run(a, b), orc.process_pair(a, b, 32, 16, 16, threads=16), f"direct32-4096-{np.dtype(dtype).name}")
+
+
+def test_direct32_refuses_float_frames(gpu_ctx):
+    for dtype in (np.float32, np.float64):
+        with pytest.raises(oa.OpivError) as e:
+            oa.Plan(gpu_ctx, 256, 256, 32, dtype=dtype, variant=D)
+        assert e.value.code == 6
+    with pytest.raises(oa.OpivError) as e:
+        oa.Plan(gpu_ctx, 256, 256, 64, dtype=np.uint8, variant=D)
+    assert e.value.code == 6
