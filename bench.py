@@ -402,7 +402,8 @@ def bench_batch(steps: int, warmup: int):
                                    "Lamb-Oseen vortex r_c 40 px peak 5 px, passes 64/32/16 at 50%",
                        "pairs": len(pairs), "pixels_per_frame_total": npx, "windows": batch.count,
                        "windows_per_pass": per_pass,
-                       "shapes": [batch.shape(p) for p in range(3)], "create_and_tune_s": round(tune_s, 2)},
+                       "shapes": [batch.shape(p) for p in range(3)], "single_launch": batch.form(),
+                       "create_and_tune_s": round(tune_s, 2)},
             "roofline": {"bound": "hbm", "achieved": round((3 * 2 * 4 * npx + batch.count * ALG_RESULT_BYTES)
                                                            / (ms * 1e-3) / 1e9, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",

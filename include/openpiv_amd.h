@@ -282,8 +282,14 @@ typedef struct opiv_batch opiv_batch;
  * pass's launch shape (threads per block, persistent blocks) from the tuning
  * cache, or by timing the candidates on these frames when the cache has no
  * entry.  The shape never changes a window's arithmetic, so results do not
- * depend on it.  The cache is the JSON file $OPIV_TUNE_CACHE, else
- * $HOME/.cache/openpiv_amd/tune.json; OPIV_TUNE=0 disables timing. */
+ * depend on it.  With two passes or more and every window at most 64 x 64
+ * it also times all passes in one persistent launch (the multi-pass kernel)
+ * against one launch per pass and keeps the faster form;
+ * OPIV_BATCH_FORM=single|perpass forces one.  The cache is the text file $OPIV_TUNE_CACHE, else
+ * $HOME/.cache/openpiv_amd/tune.txt (one line per key: arch, window, pixel
+ * type and log2 window count -> threads and blocks per CU x 4; arch, "form",
+ * pixel type, passes, log2 windows -> single launch and its blocks per CU x
+ * 4); OPIV_TUNE=0 disables timing. */
 int opiv_batch_create(opiv_ctx* ctx, const opiv_batch_desc* desc, const opiv_pair_desc* pairs,
                       size_t npairs, opiv_batch** out);
 int opiv_batch_destroy(opiv_batch* batch);
@@ -296,6 +302,9 @@ int opiv_batch_offsets(const opiv_batch* batch, size_t* offsets);
 int opiv_batch_set_pairs(opiv_batch* batch, const opiv_pair_desc* pairs, size_t npairs);
 /* the chosen launch shape of pass p: threads per block and block count */
 int opiv_batch_shape(const opiv_batch* batch, uint32_t pass, int32_t* threads, int32_t* blocks);
+/* the launch form: *single = 1 when every pass runs in one persistent launch
+ * of *blocks blocks, 0 for one launch per pass */
+int opiv_batch_form(const opiv_batch* batch, int32_t* single, int32_t* blocks);
 /* Enqueue every pass as one graph launch on the context stream; d_out is
  * device memory for opiv_batch_window_count results.  Does not synchronise. */
 int opiv_batch_run(opiv_batch* batch, opiv_window_result* d_out, size_t n);

@@ -1,10 +1,11 @@
 // batch.cpp -- many frame pairs and several window passes per launch.
 //
-// Each pass (one window size over every pair) is a single persistent launch
-// of the fused kernel reading its pairs from a device table; the passes of a
-// batch are captured once into a HIP graph and replayed by opiv_batch_run.
-// Launch shapes (threads per block, persistent block count) come from a
-// tuning cache or are timed on the batch's own frames.  The shape never
+// Each pass (one window size over every pair) is one launch of the fused
+// kernel reading its pairs from a device table, or every pass goes into one
+// persistent launch of the multi-pass kernel; either way the batch's work is
+// captured once into a HIP graph and replayed by opiv_batch_run.  Launch
+// shapes (threads per block, persistent block count per pass) and the launch
+// form come from a tuning cache or are timed on the batch's own frames.  The shape never
 // changes a window's arithmetic: which windows share a packed inverse FFT is
 // fixed by the window numbering, not by the grid, so results are
 // bit-identical for every shape.
@@ -67,6 +68,49 @@ bool cache_lookup(const std::string& key, int cu, Shape& s)
     return false;
 }
 
+// the launch form's line: "<arch> form <dtype> <w0>/<w1>/.. <log2 windows> <single> <blocks per CU x 4>"
+std::string form_key(const std::string& arch, const opiv_batch_desc& d, size_t nwin)
+{
+    std::ostringstream k;
+    k << arch << " form " << d.dtype << " ";
+    for (uint32_t ps = 0; ps < d.npasses; ++ps) k << (ps ? "/" : "") << d.win[ps];
+    k << " " << (int)std::lround(std::log2((double)std::max<size_t>(1, nwin)));
+    return k.str();
+}
+
+bool cache_lookup_raw(const std::string& key, int& v0, int& v1)
+{
+    std::ifstream is(cache_path());
+    std::string line;
+    while (std::getline(is, line)) {
+        if (line.rfind(key + " ", 0) != 0) continue;
+        std::istringstream r(line.substr(key.size() + 1));
+        if (r >> v0 >> v1) return true;
+    }
+    return false;
+}
+
+void cache_store_line(const std::string& key, const std::string& value)
+{
+    const std::string path = cache_path();
+    std::vector<std::string> keep;
+    {
+        std::ifstream is(path);
+        std::string line;
+        while (std::getline(is, line))
+            if (!line.empty() && line.rfind(key + " ", 0) != 0) keep.push_back(line);
+    }
+    for (size_t pos = path.find('/', 1); pos != std::string::npos; pos = path.find('/', pos + 1))
+        mkdir(path.substr(0, pos).c_str(), 0755);
+    const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+    std::ofstream os(tmp);
+    for (const auto& l : keep) os << l << "\n";
+    os << key << " " << value << "\n";
+    os.close();
+    if (os) std::rename(tmp.c_str(), path.c_str());
+    else std::remove(tmp.c_str());
+}
+
 void cache_store(const std::string& key, int cu, const Shape& s)
 {
     const std::string path = cache_path();
@@ -98,6 +142,8 @@ struct opiv_batch {
     std::vector<size_t> offsets;          // npasses * npairs + 1
     std::vector<int32_t> pass_windows;
     std::vector<Shape> shape;
+    bool single = false;                  // every pass in one persistent launch (corr_fft_multi.hip)
+    int single_blocks = 0;                // its grid
     DevBuf d_table;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -201,6 +247,79 @@ int tune(opiv_batch* b, uint32_t pass, opiv_window_result* scratch)
     return OPIV_OK;
 }
 
+// whether the batch has a single-launch form: two passes or more, every
+// window 8 to 64
+bool single_possible(const opiv_batch* b)
+{
+    if (b->desc.npasses < 2) return false;
+    for (uint32_t ps = 0; ps < b->desc.npasses; ++ps)
+        if (b->desc.win[ps] > 64) return false;
+    return true;
+}
+
+// the single-launch kernel's parameters: pass p writes from out + its offset
+opiv::MultiPassParams multi_params(const opiv_batch* b, opiv_window_result* out)
+{
+    opiv::MultiPassParams mp{};
+    mp.npasses = (int32_t)b->desc.npasses;
+    for (uint32_t ps = 0; ps < b->desc.npasses; ++ps) {
+        mp.pass[ps] = pass_params(b, ps, out + b->offsets[(size_t)ps * b->npairs]);
+        mp.win[ps] = (int32_t)b->desc.win[ps];
+    }
+    return mp;
+}
+
+// every pass as the batch launches it: one launch per pass with its shape,
+// or the single launch
+hipError_t enqueue_passes(const opiv_batch* b, opiv_window_result* out, bool single, int single_blocks,
+                          hipStream_t stream)
+{
+    if (single) return opiv::launch_corr_fft_multipass(multi_params(b, out), b->desc.dtype, single_blocks, stream);
+    for (uint32_t ps = 0; ps < b->desc.npasses; ++ps) {
+        const hipError_t e = opiv::launch_corr_fft_shape(pass_params(b, ps, out + b->offsets[(size_t)ps * b->npairs]),
+                                                         (int)b->desc.win[ps], b->desc.dtype, false,
+                                                         b->shape[ps].nt, b->shape[ps].blocks, stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// time the per-pass launches (with their chosen shapes) against the single
+// launch at 2 and 4 blocks per CU, back to back on the batch's own frames,
+// and keep the fastest form
+int tune_form(opiv_batch* b, opiv_window_result* scratch)
+{
+    const int cu = b->ctx->cu_count > 0 ? b->ctx->cu_count : 256;
+    struct Form {
+        bool single;
+        int blocks;
+    };
+    const Form forms[] = {{false, 0}, {true, 2 * cu}, {true, 4 * cu}};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    Form pick = forms[0];
+    for (const auto& f : forms) {
+        HIP_TRY(enqueue_passes(b, scratch, f.single, f.blocks, b->ctx->stream));   // warm-up
+        HIP_TRY(hipEventRecord(e0, b->ctx->stream));
+        for (int r = 0; r < 3; ++r) HIP_TRY(enqueue_passes(b, scratch, f.single, f.blocks, b->ctx->stream));
+        HIP_TRY(hipEventRecord(e1, b->ctx->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            pick = f;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->single = pick.single;
+    b->single_blocks = pick.blocks;
+    return OPIV_OK;
+}
+
 void drop_graph(opiv_batch* b)
 {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -250,6 +369,30 @@ int opiv_batch_create(opiv_ctx* ctx, const opiv_batch_desc* desc, const opiv_pai
         if (rc == OPIV_OK) rc = tune(b, ps, reinterpret_cast<opiv_window_result*>(scratch.p));
         if (rc != OPIV_OK) break;
         cache_store(key, ctx->cu_count, b->shape[ps]);
+    }
+    // the launch form: OPIV_BATCH_FORM=single|perpass forces it; otherwise
+    // the cache, or timing both forms when tuning is allowed
+    if (rc == OPIV_OK && single_possible(b)) {
+        const char* form_env = std::getenv("OPIV_BATCH_FORM");
+        const std::string fk = form_key(ctx->arch, *desc, b->offsets.back());
+        const int cu = ctx->cu_count > 0 ? ctx->cu_count : 256;
+        int v0 = 0, v1 = 0;
+        if (form_env && std::string(form_env) == "single") {
+            b->single = true;
+            b->single_blocks = 2 * cu;
+        } else if (form_env && std::string(form_env) == "perpass") {
+            b->single = false;
+        } else if (cache_lookup_raw(fk, v0, v1) && (v0 == 0 || (v0 == 1 && v1 > 0))) {
+            b->single = v0 == 1;
+            b->single_blocks = v1 * cu / 4;
+        } else if (may_time) {
+            hipError_t e = scratch.ensure(b->offsets.back() * sizeof(opiv_window_result));
+            if (e != hipSuccess) rc = hip_fail(e, "tuning scratch");
+            if (rc == OPIV_OK) rc = tune_form(b, reinterpret_cast<opiv_window_result*>(scratch.p));
+            if (rc == OPIV_OK)
+                cache_store_line(fk, std::to_string(b->single ? 1 : 0) + " " +
+                                         std::to_string(b->single ? b->single_blocks * 4 / cu : 0));
+        }
     }
     (void)hipStreamSynchronize(ctx->stream);
     scratch.release();
@@ -311,6 +454,14 @@ int opiv_batch_shape(const opiv_batch* batch, uint32_t pass, int32_t* threads, i
     return OPIV_OK;
 }
 
+int opiv_batch_form(const opiv_batch* batch, int32_t* single, int32_t* blocks)
+{
+    if (!batch || !single || !blocks) return fail(OPIV_E_INVALID, "opiv_batch_form: null argument");
+    *single = batch->single ? 1 : 0;
+    *blocks = batch->single ? batch->single_blocks : 0;
+    return OPIV_OK;
+}
+
 int opiv_batch_run(opiv_batch* batch, opiv_window_result* d_out, size_t n)
 {
     if (!batch || !d_out) return fail(OPIV_E_INVALID, "opiv_batch_run: null argument");
@@ -323,12 +474,7 @@ int opiv_batch_run(opiv_batch* batch, opiv_window_result* d_out, size_t n)
     if (!batch->exec || batch->graph_out != d_out) {
         drop_graph(batch);
         HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t le = hipSuccess;
-        for (uint32_t ps = 0; ps < batch->desc.npasses && le == hipSuccess; ++ps)
-            le = opiv::launch_corr_fft_shape(pass_params(batch, ps, d_out + batch->offsets[(size_t)ps * batch->npairs]),
-                                             (int)batch->desc.win[ps],
-                                             batch->desc.dtype, false, batch->shape[ps].nt,
-                                             batch->shape[ps].blocks, ctx->stream);
+        const hipError_t le = enqueue_passes(batch, d_out, batch->single, batch->single_blocks, ctx->stream);
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
         if (le != hipSuccess) {

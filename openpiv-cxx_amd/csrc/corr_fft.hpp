@@ -42,6 +42,18 @@ hipError_t launch_corr_fft(const CorrParams& p, int win, int dtype, bool planes,
 hipError_t launch_corr_fft_shape(const CorrParams& p, int win, int dtype, bool planes, int nt, int blocks,
                                  hipStream_t stream);
 
+// Every pass of a batch in one persistent launch (corr_fft_multi.hip):
+// pass k is a batch-mode CorrParams (pair table) of window size win[k] in
+// [8, 64]; the grid's waves work through the passes in order, each XCD
+// taking a contiguous share of every pass.  blocks: the persistent grid
+// (two 256-thread blocks fit per CU).
+struct MultiPassParams {
+    CorrParams pass[4];
+    int32_t win[4];
+    int32_t npasses;
+};
+hipError_t launch_corr_fft_multipass(const MultiPassParams& mp, int dtype, int blocks, hipStream_t stream);
+
 // 128 x 128 windows (corr_fft128.hip): one workgroup of 256 threads per
 // window, single pair or batch (p.pairs), fused or plane mode
 hipError_t launch_corr_fft128(const CorrParams& p, int dtype, bool planes, hipStream_t stream);

@@ -58,7 +58,7 @@ EXPORTS = (
     "opiv_run_pair", "opiv_run_pair_async",
     "opiv_correlate_windows", "opiv_auto_correlate_windows", "opiv_fft_transform_windows",
     "opiv_batch_create", "opiv_batch_destroy", "opiv_batch_window_count", "opiv_batch_offsets",
-    "opiv_batch_set_pairs", "opiv_batch_shape", "opiv_batch_run",
+    "opiv_batch_set_pairs", "opiv_batch_shape", "opiv_batch_form", "opiv_batch_run",
     "opiv_vjp_pair", "opiv_vjp_pair_async", "opiv_vjp_pair_async_noalloc", "opiv_variance_pair",
     "opiv_average_subtract",
     "opiv_image_probe", "opiv_image_read",
@@ -155,6 +155,7 @@ def load() -> ctypes.CDLL:
     lib.opiv_batch_offsets.argtypes = [vp, vp]
     lib.opiv_batch_set_pairs.argtypes = [vp, ctypes.POINTER(PairDesc), sz]
     lib.opiv_batch_shape.argtypes = [vp, u32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    lib.opiv_batch_form.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.opiv_batch_run.argtypes = [vp, vp, sz]
     lib.opiv_vjp_pair.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.opiv_vjp_pair_async.argtypes = [vp, vp, vp, vp, vp, vp, vp]
@@ -489,6 +490,13 @@ class Batch:
         t, b = ctypes.c_int32(), ctypes.c_int32()
         _check(self._lib.opiv_batch_shape(self.handle, p, ctypes.byref(t), ctypes.byref(b)))
         return t.value, b.value
+
+    def form(self) -> tuple[bool, int]:
+        """(True, blocks) when every pass runs in one persistent launch of
+        ``blocks`` blocks, (False, 0) for one launch per pass."""
+        single, blocks = ctypes.c_int32(), ctypes.c_int32()
+        _check(self._lib.opiv_batch_form(self.handle, ctypes.byref(single), ctypes.byref(blocks)))
+        return bool(single.value), blocks.value
 
     def run(self, out_ptr: int) -> None:
         """Enqueue all passes (one graph launch); out_ptr: device, ``count`` results."""

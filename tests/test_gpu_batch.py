@@ -182,3 +182,57 @@ def test_batch_with_128_pass(gpu_ctx, batch_setup):
             a, b = frames[i]
             plan = oa.Plan(gpu_ctx, a.shape[1], a.shape[0], win, overlap=0.5, dtype=a.dtype)
             assert batch.block(res, p, j).tobytes() == plan.run(a, b).tobytes(), (win, i)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.uint8, np.uint16, np.float64])
+def test_single_launch_equals_per_pass(gpu_ctx, monkeypatch, dtype):
+    """every pass in one persistent launch (the multi-pass kernel) against one
+    launch per pass, passes 8 / 64 / 16 / 32 over the uneven pairs: the same
+    per-window code runs either way, so the results are bit-identical"""
+    frames = make_pairs(dtype, seed0=300)
+    bufs = [(oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)) for a, b in frames]
+    pairs = [(da.ptr, db.ptr, a.shape[1], a.shape[0]) for (a, b), (da, db) in zip(frames, bufs)]
+    wins = (8, 64, 16, 32)
+    res = {}
+    for form in ("perpass", "single"):
+        monkeypatch.setenv("OPIV_BATCH_FORM", form)
+        batch = oa.Batch(gpu_ctx, pairs, wins, dtype=dtype)
+        single, blocks = batch.form()
+        assert single == (form == "single") and (blocks > 0) == single, (form, single, blocks)
+        res[form] = run_batch(batch)[1]
+        say(f"{np.dtype(dtype).name} {form}: {batch.count} windows, blocks {blocks}")
+    assert res["single"].tobytes() == res["perpass"].tobytes()
+
+
+def test_single_launch_options_and_cache(gpu_ctx, batch_setup, tmp_path, monkeypatch):
+    """limit_search and REAL_COMPAT reach the single launch; the cached form
+    line (single launch, blocks per CU x 4) is obeyed without timing"""
+    frames, bufs, pairs = batch_setup
+    for kw in ({"limit_search": True}, {"variant": oa.REAL_COMPAT}):
+        res = {}
+        for form in ("perpass", "single"):
+            monkeypatch.setenv("OPIV_BATCH_FORM", form)
+            res[form] = run_batch(oa.Batch(gpu_ctx, pairs, WINS, **kw))[1]
+        assert res["single"].tobytes() == res["perpass"].tobytes(), kw
+    monkeypatch.setenv("OPIV_BATCH_FORM", "perpass")
+    plain = run_batch(oa.Batch(gpu_ctx, pairs, WINS))[1]
+    monkeypatch.setenv("OPIV_BATCH_FORM", "single")
+    forced = oa.Batch(gpu_ctx, pairs, WINS).form()
+    monkeypatch.delenv("OPIV_BATCH_FORM")
+    n = sum((1 + (w - win) // (win // 2)) * (1 + (h - win) // (win // 2)) for win in WINS for w, h in SIZES)
+    cache = tmp_path / "tune.txt"
+    cache.write_text(f"gfx950 form 2 64/32/16 {int(round(np.log2(n)))} 1 16\n")
+    monkeypatch.setenv("OPIV_TUNE_CACHE", str(cache))
+    monkeypatch.setenv("OPIV_TUNE", "0")
+    cached = oa.Batch(gpu_ctx, pairs, WINS)
+    assert cached.form() == (True, 2 * forced[1]), (cached.form(), forced)
+    assert run_batch(cached)[1].tobytes() == plain.tobytes()
+
+
+def test_128_pass_stays_per_pass(gpu_ctx, batch_setup, monkeypatch):
+    """a 128 x 128 pass runs one workgroup per window: the batch keeps one
+    launch per pass even when the single launch is asked for"""
+    frames, bufs, pairs = batch_setup
+    sub = [p for p, (a, _) in zip(pairs, frames) if min(a.shape) >= 128]
+    monkeypatch.setenv("OPIV_BATCH_FORM", "single")
+    assert oa.Batch(gpu_ctx, sub, (128, 32)).form() == (False, 0)
