@@ -13,10 +13,11 @@
 // (lanes 16 g .. 16 g + 15) window row y = 2c + g / 2 and columns
 // 16 (g % 2) .. + 15; the same k slots in A and B, so the MFMA sums exactly
 // the window's products.
-//   * A (tile T, lane i = lane % 16): 16 bytes of row (y - 16 T - i) % 32 of
-//     a.  From one chunk to the next y grows by 2, so lane i takes lane
-//     i - 2's operand (DPP row_ror:2 within the 16-lane row) and lanes 0, 1
-//     take lanes 14, 15 of the other tile: the A operands never return to LDS.
+//   * A (tile T, lane i = lane % 16): output row sy = 2 i + T, 16 bytes of
+//     row (y - sy) % 32 of a.  From one chunk to the next y grows by 2, which
+//     is what row sy - 2, lane i - 1 of the same tile, held (lane 0 takes
+//     lane 15): one DPP row_ror:1 per operand register, and the A operands
+//     never return to LDS.
 //   * B (tile U): 16 bytes of row y of b starting at column
 //     (16 (g % 2) + 16 U + i) % 32, read from one of four byte-shifted copies
 //     of the doubled row so the run is dword-aligned.
@@ -28,10 +29,12 @@
 // 65536 C_hh + 256 (C_hl + C_lh) + C_ll.  The integer correlation is rounded
 // once, to FP32, for the peak search.
 //
-// A wave correlates two windows in turn, one per 32-lane group, keeps their
-// planes in registers (16 values per lane each), then moves them through the
-// wave's LDS slice into the column layout of the FFT kernel's tail
-// (finish_windows: lane t of a group holds column t of its window's plane).
+// A wave correlates four windows in turn, two per 32-lane group, keeps
+// their planes in registers (16 values per lane each), then moves them, one
+// round of two windows at a time, through the wave's LDS slice into the
+// column layout of the FFT kernel's tail (finish_windows: lane t of a group
+// holds column t of its windows' planes), whose fits, reductions and logs
+// then serve two windows per group.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -52,15 +55,18 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int W = 32;
 constexpr int WAVES = 4;                   // waves per block, one task each
-constexpr int ROUNDS = 1, WPW = 2, PER = ROUNDS * WPW;
+constexpr int ROUNDS = 2, WPW = 2, PER = ROUNDS * WPW;
 
 // LDS staging of one window, per byte plane: a as 32 rows of 8 dwords
 // (padded to 12: the rows of one b128 read group start in distinct bank
-// quads), b as four copies of the doubled rows (16 dwords each, copy s
-// shifted by s bytes: byte m of row y holds b[y][(m + s) % 32]).
+// quads), b as four copies of its rows extended periodically to 48 bytes
+// (12 dwords, enough for a 16-byte run from any of the 32 starts; copy s
+// shifted by s bytes: byte m of row y holds b[y][(m + s) % 32]).  The copies
+// start 8 banks apart, so the 32 lanes of one B-operand read pass (4 copies
+// x 2 column halves x 4 dword offsets) hit 32 distinct banks.
 constexpr int APITCH = 12;
-constexpr int BROW = 16;
-constexpr int BCOPY = W * BROW + 4;
+constexpr int BROW = 12;
+constexpr int BCOPY = W * BROW + 8;
 struct PlaneStage {
     uint32_t a[W * APITCH];
     uint32_t b[4][BCOPY];
@@ -78,9 +84,9 @@ constexpr int lds_words()
     return stage > SLICE ? stage : SLICE;
 }
 
-__device__ __forceinline__ uint32_t dpp_ror2(uint32_t v)
+__device__ __forceinline__ uint32_t dpp_ror1(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x122, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x121, 0xf, 0xf, false);
 }
 
 // 16 consecutive pixels of one frame row (as bytes for 8-bit frames, as
@@ -183,7 +189,8 @@ void corr_direct32_kernel(CorrParams p)
     };
 
     // the four windows' quadrant-swapped planes, MFMA layout: pv[i][T][U][r]
-    // = S[16 (1 - T) + 4 g + r][16 (1 - U) + i16]
+    // = S[(8 g + 2 r + T + 16) % 32][16 (1 - U) + i16] (tile row m = 4 g + r
+    // is correlation row 2 m + T)
     float pv[PER][2][2][4];
     Run<DT> ra[2], rb[2];
     fetch(std::integral_constant<int, 0>{}, ra[0], rb[0]);
@@ -223,8 +230,9 @@ void corr_direct32_kernel(CorrParams p)
                     wv[j] = s == 0 ? pb[k][j] : __builtin_amdgcn_alignbyte(hi, pb[k][j], s);
                 });
                 const uint4 v = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+                // dwords 0..7 of the row, then 8..11 repeat 0..3
                 *reinterpret_cast<uint4*>(&st[k].b[s][srow * BROW + 4 * shalf]) = v;
-                *reinterpret_cast<uint4*>(&st[k].b[s][srow * BROW + 4 * shalf + 8]) = v;
+                if (shalf == 0) *reinterpret_cast<uint4*>(&st[k].b[s][srow * BROW + 8]) = v;
             });
         });
         wave_sync();
@@ -235,7 +243,7 @@ void corr_direct32_kernel(CorrParams p)
             constexpr int k = decltype(K)::value;
             sfor<2>([&](auto Tt) {
                 constexpr int T = decltype(Tt)::value;
-                const int row = ((g >> 1) - 16 * T - i16) & (W - 1);
+                const int row = ((g >> 1) - 2 * i16 - T) & (W - 1);
                 const uint4 v = *reinterpret_cast<const uint4*>(&st[k].a[row * APITCH + 4 * (g & 1)]);
                 A[k][T][0] = v.x;
                 A[k][T][1] = v.y;
@@ -251,7 +259,6 @@ void corr_direct32_kernel(CorrParams p)
                 });
             });
         });
-        const bool lo2 = i16 < 2;
 #pragma unroll 2
         for (int c = 0; c < 16; ++c) {
             const int y = 2 * c + (g >> 1);
@@ -282,14 +289,14 @@ void corr_direct32_kernel(CorrParams p)
             });
             // the next chunk's A operands (the last chunk's rotation is unused)
             sfor<NP>([&](auto K) {
-                constexpr int k = decltype(K)::value;
-                sfor<4>([&](auto D) {
-                    constexpr int d = decltype(D)::value;
-                    const uint32_t t0 = dpp_ror2(A[k][0][d]), t1 = dpp_ror2(A[k][1][d]);
-                    A[k][0][d] = lo2 ? t1 : t0;
-                    A[k][1][d] = lo2 ? t0 : t1;
+                sfor<2>([&](auto Tt) {
+                    sfor<4>([&](auto D) {
+                        uint32_t& v = A[decltype(K)::value][decltype(Tt)::value][decltype(D)::value];
+                        v = dpp_ror1(v);
+                    });
                 });
             });
+
         }
 
         // ---- the integer correlation, rounded once to FP32 ----
@@ -319,11 +326,12 @@ void corr_direct32_kernel(CorrParams p)
     });
 
     // ---- planes to the column layout of the FFT tail: lane (slot, t) reads
-    // column t of its two windows' planes, one round at a time ----
+    // column t of its windows' planes, one round at a time, the last round
+    // first, so that the slice still holds round 0 for the tail ----
     float s[ROUNDS][W];
-    sfor<ROUNDS>([&](auto R) {
-        constexpr int r = decltype(R)::value;
-        wave_sync();                                        // the last window's staging reads are done
+    sfor<ROUNDS>([&](auto Rq) {
+        constexpr int r = ROUNDS - 1 - decltype(Rq)::value;
+        wave_sync();                                        // earlier readers of the slice are done
         sfor<WPW>([&](auto Sl) {
             constexpr int sl = decltype(Sl)::value;
             float* pl = slice + sl * W * (W + 1);
@@ -334,7 +342,7 @@ void corr_direct32_kernel(CorrParams p)
                     const int col = 16 * (1 - U) + i16;
                     sfor<4>([&](auto Rr) {
                         constexpr int rr = decltype(Rr)::value;
-                        pl[col * (W + 1) + 16 * (1 - T) + 4 * g + rr] = pv[r * WPW + sl][T][U][rr];
+                        pl[col * (W + 1) + ((8 * g + 2 * rr + T + W / 2) & (W - 1))] = pv[r * WPW + sl][T][U][rr];
                     });
                 });
             });
@@ -348,10 +356,22 @@ void corr_direct32_kernel(CorrParams p)
     // unnormalised inverse FFT), an exact power of two that the tail restores
     float unscale[ROUNDS];
     sfor<ROUNDS>([&](auto R) { unscale[decltype(R)::value] = (float)(W * W); });
-    // the slice still holds both groups' planes: every plane is readable
-    const float* pl = slice + slot * W * (W + 1);
-    finish_windows<W, ROUNDS, PLANES, false>(p, s, unscale, geo, active, lane, t, [](auto) {},
-                                             [&](int, int row, int col) { return pl[col * (W + 1) + row]; });
+    // the slice holds one round's planes (both groups): round 0 is there,
+    // round 1 goes back from the column layout when the tail asks for it
+    float* pl = slice + slot * W * (W + 1);
+    auto stage = [&](auto R) {
+        constexpr int r = decltype(R)::value;
+        if constexpr (r > 0) {
+            wave_sync();                                    // round 0's readers are done
+            sfor<W>([&](auto Hh) {
+                constexpr int h = decltype(Hh)::value;
+                pl[t * (W + 1) + h] = s[r][h];
+            });
+            wave_sync();
+        }
+    };
+    finish_windows<W, ROUNDS, PLANES, true>(p, s, unscale, geo, active, lane, t, stage,
+                                            [&](int, int row, int col) { return pl[col * (W + 1) + row]; });
 }
 
 template <int DT>
