@@ -267,9 +267,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
     // (y - i16) mod 16 of a' (chunk c + 1's is chunk c's of lane i16 - 4: DPP
     // row_ror:4), the B operand row y of b' rotated by i16, read from one of
     // four byte-shifted copies of the doubled row so that it is dword-aligned.
-    // B rows are padded to 16 dwords and the copies offset by 260 dwords, so
-    // the 64 lanes of a B read hit 64 distinct banks. ----
-    constexpr int B8ROW = 16, B8COPY = 16 * B8ROW + 4;
+    // B rows are padded to 12 dwords and the copies start 8 banks apart
+    // (200 dwords), so the 32 lanes of each pass of a B read (4 copies x 2
+    // rows x 4 dword offsets) and of a staging store (8 rows x 4 dwords) hit
+    // 32 distinct banks. ----
+    constexpr int B8ROW = 12, B8COPY = 16 * B8ROW + 8;
     uint32_t (&a8)[W][4] = *reinterpret_cast<uint32_t (*)[W][4]>(&lds[wave][0]);
     uint32_t (&b8)[4][B8COPY] = *reinterpret_cast<uint32_t (*)[4][B8COPY]>(&lds[wave][W * 16]);
     static_assert(W * 16 + 4 * B8COPY * 4 <= BYTES, "8-bit staging fits the wave's LDS");
