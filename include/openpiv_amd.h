@@ -52,14 +52,17 @@ typedef enum opiv_variant {
      * column 0 zero */
     OPIV_REAL_COMPAT = 1,
     /* the circular correlation computed directly (no FFT) as an im2col GEMM
-     * on MFMA: u8 frames exactly on int8 MFMA (pixels offset by 128), u16
-     * exact bf16 products, f32/f64 a three-term bf16 split, FP32
-     * accumulation; 16x16 windows */
+     * on MFMA.  16x16 windows: u8 frames exactly on int8 MFMA (pixels
+     * offset by 128), u16 exact bf16 products, f32/f64 a three-term bf16
+     * split, FP32 accumulation.  32x32 windows of u8 and u16 frames: exactly
+     * on int8 MFMA (16-bit pixels as two offset byte planes, four int8
+     * products).  Other sizes and types: OPIV_E_UNSUPPORTED. */
     OPIV_DIRECT_BF16_SPLIT = 2,
     /* the faster of OPIV_CIRCULAR_FFT_FP32 and OPIV_DIRECT_BF16_SPLIT for
-     * the plan's window size and pixel type as measured on MI355X (same
-     * semantics): the direct path for 8-bit frames at 16x16, the FFT path
-     * otherwise */
+     * the plan's window size and pixel type as measured on MI355X at
+     * 4096x4096 (same semantics): the direct path for 8-bit frames at 16x16
+     * (68.5 against 74.9 us), the FFT path otherwise (8-bit at 32x32: 81.6
+     * against 88.0 us) */
     OPIV_AUTO = 3
 } opiv_variant;
 
@@ -92,7 +95,7 @@ typedef struct opiv_plan_desc {
     uint32_t img_w, img_h;   /* frame size in pixels */
     size_t pitch;            /* elements per frame row (0 = img_w) */
     int32_t dtype;           /* opiv_dtype */
-    uint32_t win;            /* square window side, power of 2 in [8, 64] */
+    uint32_t win;            /* square window side, power of 2 in [8, 128] */
     uint32_t step_x, step_y; /* grid offsets in pixels (grid.impl.h:24-61); 0 = use overlap */
     double overlap;          /* process -o: step = (uint32)(win * overlap) (grid.impl.h:10-22) */
     int32_t variant;         /* opiv_variant */
@@ -143,9 +146,9 @@ int opiv_run_pair_async(opiv_plan* plan, const void* d_a, const void* d_b,
 
 /* correlator_t of main.cpp:153 for a batch of n windows:
  * FFT::cross_correlate (fft.h:188-206) or cross_correlate_real
- * (fft.h:208-227) of stacked win*win FP64 windows a[i], b[i]; writes the
- * quadrant-swapped real planes (FP64, computed in FP32).  Host or device
- * pointers. */
+ * (fft.h:208-227) of stacked win*win FP64 windows a[i], b[i] (win 8 to
+ * 128); writes the quadrant-swapped real planes (FP64, computed in FP32).
+ * Host or device pointers. */
 int opiv_correlate_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t variant,
                            const double* a, const double* b, double* planes);
 
@@ -161,8 +164,8 @@ typedef enum opiv_direction {
 } opiv_direction;
 
 /* FFT::transform (fft.h:86-119): un-normalised 2-D DFT of n stacked
- * win*win complex windows (interleaved re, im FP64, row-major), computed in
- * FP32.  in == out is allowed.  Host or device pointers. */
+ * win*win complex windows (interleaved re, im FP64, row-major; win 8 to
+ * 64), computed in FP32.  in == out is allowed.  Host or device pointers. */
 int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t direction,
                                const double* in, double* out);
 
@@ -213,7 +216,8 @@ int opiv_average_subtract(opiv_ctx* ctx, const void* const* frames, size_t n, ui
  * the pixels of both frames, with the integer peak held fixed.  The adjoint
  * of the circular correlation at the five stencil shifts is again a
  * correlation; the fit's derivatives use FP64 logs of stencil values
- * recomputed as direct dot products.  Only OPIV_CIRCULAR_FFT_FP32 plans.
+ * recomputed as direct dot products.  Only OPIV_CIRCULAR_FFT_FP32 plans of
+ * windows 8 to 64.
  * fwd: the plan's results of opiv_run_pair for the same frames (NULL = run
  * the forward pass here).  Windows flagged FEWER_THAN_2_PEAKS or
  * NONFINITE_FIT contribute nothing.  All pointers host or device; returns
@@ -270,7 +274,7 @@ typedef struct opiv_pair_desc {
 typedef struct opiv_batch_desc {
     int32_t dtype;           /* opiv_dtype shared by all pairs */
     uint32_t npasses;        /* 1..4 */
-    uint32_t win[4];         /* square window per pass, power of 2 in [8, 64] */
+    uint32_t win[4];         /* square window per pass, power of 2 in [8, 128] */
     double overlap[4];       /* process -o per pass (grid.impl.h:10-22) */
     int32_t variant;         /* opiv_variant */
     int32_t limit_search;
