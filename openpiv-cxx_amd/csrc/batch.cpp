@@ -210,6 +210,21 @@ int upload_table(opiv_batch* b)
     return OPIV_OK;
 }
 
+// the two timing events of a tuning run, destroyed on every return path
+struct TimingEvents {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t create()
+    {
+        const hipError_t e = hipEventCreate(&e0);
+        return e != hipSuccess ? e : hipEventCreate(&e1);
+    }
+    ~TimingEvents()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 // time each candidate shape on the batch's own frames and keep the fastest
 int tune(opiv_batch* b, uint32_t pass, opiv_window_result* scratch)
 {
@@ -220,9 +235,9 @@ int tune(opiv_batch* b, uint32_t pass, opiv_window_result* scratch)
     }
     const std::vector<Shape> cands = {{256, 0}, {64, 0}, {256, 2 * cu}, {256, 4 * cu}, {64, 8 * cu}, {64, 16 * cu}};
     const opiv::CorrParams p = pass_params(b, pass, scratch);
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    TimingEvents ev;
+    HIP_TRY(ev.create());
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     float best = 1e30f;
     Shape pick = cands[0];
     for (const auto& c : cands) {
@@ -241,8 +256,6 @@ int tune(opiv_batch* b, uint32_t pass, opiv_window_result* scratch)
             pick = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     b->shape[pass] = pick;
     return OPIV_OK;
 }
@@ -295,9 +308,9 @@ int tune_form(opiv_batch* b, opiv_window_result* scratch)
         int blocks;
     };
     const Form forms[] = {{false, 0}, {true, 2 * cu}, {true, 4 * cu}};
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    TimingEvents ev;
+    HIP_TRY(ev.create());
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     float best = 1e30f;
     Form pick = forms[0];
     for (const auto& f : forms) {
@@ -313,8 +326,6 @@ int tune_form(opiv_batch* b, opiv_window_result* scratch)
             pick = f;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     b->single = pick.single;
     b->single_blocks = pick.blocks;
     return OPIV_OK;

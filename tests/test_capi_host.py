@@ -93,3 +93,19 @@ def test_oracle_is_not_linked_by_the_product():
     assert "oracle" not in out
     text = " ".join(p.read_text() for p in (ROOT / "openpiv-cxx_amd" / "openpiv_amd").glob("*.py"))
     assert "import oracle" not in text and "liboracle" not in text
+
+
+def test_batch_entry_points_reject_null_handles():
+    """the batch entry points check their arguments before any HIP call, so
+    a null handle is an OPIV_E_INVALID error with a message, on any host"""
+    lib = oa.load()
+    i32 = ctypes.c_int32
+    a, b = i32(), i32()
+    n = ctypes.c_size_t()
+    for rc, what in [(lib.opiv_batch_form(None, ctypes.byref(a), ctypes.byref(b)), "opiv_batch_form"),
+                     (lib.opiv_batch_shape(None, 0, ctypes.byref(a), ctypes.byref(b)), "opiv_batch_shape"),
+                     (lib.opiv_batch_window_count(None, ctypes.byref(n)), "opiv_batch_window_count"),
+                     (lib.opiv_batch_run(None, None, 0), "opiv_batch_run")]:
+        assert rc == 1, (what, rc)
+        assert oa.last_error(), what
+    assert lib.opiv_batch_destroy(None) == 0
