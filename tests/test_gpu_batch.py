@@ -135,13 +135,13 @@ def test_batch_errors(gpu_ctx, batch_setup):
         oa.Batch(gpu_ctx, [(pairs[0][0], pairs[0][1], 40, 40)], (64,))
 
 
-def test_multipass_workload_full_size(gpu_ctx):
+def test_multipass_workload_full_size(gpu_ctx, monkeypatch):
     """The benchmark's batch exactly: 24 seeded pairs of 512^2 .. 4096^2 and
     1280 x 800 with Lamb-Oseen vortices (r_c 40 px, peak 5 px), passes
     64 / 32 / 16 at 50% overlap in one graph launch.  Every window of every
-    pair and pass is bit-identical to the single-pair launch; the largest
-    pair and a 1280 x 800 pair of each pass are checked against the FP64
-    oracle window by window."""
+    pair and pass is bit-identical to the single-pair launch, in both launch
+    forms; the largest pair and a 1280 x 800 pair of each pass are checked
+    against the FP64 oracle window by window."""
     frames = list(oa.multipass_pairs(24, seed=24))
     sizes = [a.shape for a, _ in frames]
     say("multipass 24 pairs", sizes)
@@ -149,7 +149,12 @@ def test_multipass_workload_full_size(gpu_ctx):
     pairs = [(da.ptr, db.ptr, a.shape[1], a.shape[0]) for (a, _), (da, db) in zip(frames, bufs)]
     batch = oa.Batch(gpu_ctx, pairs, WINS)
     _, res = run_batch(batch)
-    say("batch windows", batch.count, "shapes", [batch.shape(p) for p in range(3)])
+    say("batch windows", batch.count, "shapes", [batch.shape(p) for p in range(3)], "form", batch.form())
+    monkeypatch.setenv("OPIV_BATCH_FORM", "single" if not batch.form()[0] else "perpass")
+    other = oa.Batch(gpu_ctx, pairs, WINS)
+    assert other.form()[0] != batch.form()[0]
+    assert run_batch(other)[1].tobytes() == res.tobytes()
+    say("both launch forms bit-identical")
     big = max(range(len(frames)), key=lambda i: frames[i][0].size)
     wide = [i for i, s in enumerate(sizes) if s == (800, 1280)]
     for p, win in enumerate(WINS):
