@@ -78,6 +78,32 @@ def test_vjp_random_cotangent_whole_frame(gpu_ctx):
         assert np.abs(got - ref).max() <= GRAD_TOL * np.abs(ref).max()
 
 
+def test_differentiation_workload_full_size(gpu_ctx):
+    """The differentiation workload at its stated size: a 1024^2 f32 pair
+    shifted by (1.6, -0.8), 32^2 windows step 16 (3969 windows), a seeded
+    cotangent on every window; the whole-frame gradients against the FP64
+    CPU adjoint and the variances (sigma 2) against the FP64 variance"""
+    a, b = frames(1024, 1024, seed=5)
+    plan = oa.Plan(gpu_ctx, 1024, 1024, 32, dtype=np.float32)
+    assert plan.count == 3969
+    fwd = plan.run(a, b)
+    tied = (fwd["flags"] & (oa.FLAG_NEAR_TIE | oa.FLAG_EXACT_TIE)) != 0
+    rng = np.random.default_rng(1)
+    cot = rng.normal(size=(plan.count, 2))
+    cot[tied] = 0
+    ga, gb = plan.vjp(a, b, cot, fwd)
+    ra, rb = orc.vjp_pair(a, b, 32, 16, 16, cot)
+    for got, ref in ((ga, ra), (gb, rb)):
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err <= GRAD_TOL, err
+    var = plan.variance(a, b, 2.0, fwd)
+    ref = orc.variance_pair(a, b, 32, 16, 16, 2.0)
+    ok = ~tied & ((fwd["flags"] & oa.FLAG_FEWER_THAN_2_PEAKS) == 0)
+    assert ok.sum() > 3900
+    assert np.allclose(var[ok], ref[ok], rtol=GRAD_TOL, atol=0)
+    say(f"1024^2 vjp vs fp64 ok, {int(tied.sum())} tied windows, variance {int(ok.sum())} windows")
+
+
 def test_variance_matches_fp64(gpu_ctx):
     a, b = frames(512, 512, seed=11)
     plan = oa.Plan(gpu_ctx, 512, 512, 32, dtype=np.float32)
