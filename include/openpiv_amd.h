@@ -165,7 +165,7 @@ typedef enum opiv_direction {
 
 /* FFT::transform (fft.h:86-119): un-normalised 2-D DFT of n stacked
  * win*win complex windows (interleaved re, im FP64, row-major; win 8 to
- * 64), computed in FP32.  in == out is allowed.  Host or device pointers. */
+ * 128), computed in FP32.  in == out is allowed.  Host or device pointers. */
 int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t direction,
                                const double* in, double* out);
 

@@ -584,7 +584,6 @@ int opiv_fft_transform_windows(opiv_ctx* ctx, uint32_t win, size_t n, int32_t di
 {
     if (!ctx || !in || !out) return fail(OPIV_E_INVALID, "transform: null argument");
     if (int rc = check_window(win)) return rc;
-    if (win > 64) return fail(OPIV_E_UNSUPPORTED, "the transform kernel takes windows up to 64x64, not %ux%u", win, win);
     if (direction != OPIV_FORWARD && direction != OPIV_REVERSE)
         return fail(OPIV_E_INVALID, "unknown direction %d", direction);
     if (n == 0) return OPIV_OK;

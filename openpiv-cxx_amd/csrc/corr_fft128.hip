@@ -37,6 +37,7 @@
 #include "corr_common.hpp"
 #include "corr_fft.hpp"
 #include "fft_reg.hpp"
+#include "transform.hpp"
 
 namespace opiv {
 
@@ -365,6 +366,77 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void co
         }
         p.out[g.out] = res;
     }
+}
+
+// FFT::transform (fft.h:86-119) of 128 x 128 windows of interleaved complex
+// FP64 values, un-normalised both ways, FP32 arithmetic: one 256-thread
+// workgroup per window, the lane pairs and transposes of the correlation
+// kernel (columns, then rows).  The reverse transform runs the forward
+// network on conjugates, IDFT(x) = conj(DFT(conj(x))).  Every thread has read
+// its window before any thread writes it, so in == out is allowed.
+template <bool FORWARD>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
+void fft128_transform_kernel(const double2* in, double2* out, int n)
+{
+    __shared__ __attribute__((aligned(16))) float plane[W * LP];
+    const int tid = threadIdx.x;
+    const int l = tid >> 1, q = tid & 1;
+    const float sg = q ? -1.f : 1.f;
+    const float cj = FORWARD ? 1.f : -1.f;                 // conjugation of the reverse transform
+    if ((int)blockIdx.x >= n) return;                      // the whole workgroup
+    const size_t base = (size_t)blockIdx.x * W * W;
+
+    // ---- column l: rows 2m + q ----
+    float2 z[H2];
+    sfor<H2>([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        const double2 v = in[base + (size_t)(2 * m + q) * W + l];
+        z[m] = make_float2((float)v.x, cj * (float)v.y);
+    });
+    fwd_line(z, q, sg);                                    // column l: ky = k + 64 q at z[brev(k)]
+
+    // ---- transpose (real parts, then imaginary parts): line l = row ky = l ----
+    auto transpose_in = [&](auto part) {
+        constexpr int pt = decltype(part)::value;
+        __syncthreads();
+        sfor<H2>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            const float2 v = z[brev_n<H2>(k)];
+            plane[(k + H2 * q) * LP + l] = pt == 0 ? v.x : v.y;
+        });
+        __syncthreads();
+    };
+    float re[H2];
+    transpose_in(std::integral_constant<int, 0>{});
+    sfor<H2>([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        re[m] = plane[l * LP + 2 * m + q];
+    });
+    transpose_in(std::integral_constant<int, 1>{});
+    sfor<H2>([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        z[m] = make_float2(re[m], plane[l * LP + 2 * m + q]);
+    });
+    fwd_line(z, q, sg);                                    // row ky = l: kx = k + 64 q at z[brev(k)]
+
+    sfor<H2>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        const float2 v = z[brev_n<H2>(k)];
+        out[base + (size_t)l * W + k + H2 * q] = make_double2((double)v.x, (double)(cj * v.y));
+    });
+}
+
+hipError_t launch_fft_transform128(const double* in, double* out, size_t n, bool forward, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    if (n > 0x7fffffffull) return hipErrorInvalidValue;
+    const auto* src = reinterpret_cast<const double2*>(in);
+    auto* dst = reinterpret_cast<double2*>(out);
+    if (forward)
+        hipLaunchKernelGGL((fft128_transform_kernel<true>), dim3((unsigned)n), dim3(NT), 0, stream, src, dst, (int)n);
+    else
+        hipLaunchKernelGGL((fft128_transform_kernel<false>), dim3((unsigned)n), dim3(NT), 0, stream, src, dst, (int)n);
+    return hipGetLastError();
 }
 
 template <int DT>

@@ -3,10 +3,11 @@
 // and exp(+i ...) reverse (fft.h:274-294 twiddle signs), for n stacked square
 // windows of interleaved complex FP64 values.  FP32 arithmetic.
 //
-// Mapping: W lanes own a window (lane t = column t); the column FFT runs in
-// VGPRs, an LDS transpose hands lane t row t for the row FFT, and a second
-// transpose lets lane t store output column t, so loads and stores are
-// coalesced across the lanes of a row.
+// Mapping (windows 8 to 64): W lanes own a window (lane t = column t); the
+// column FFT runs in VGPRs, an LDS transpose hands lane t row t for the row
+// FFT, and a second transpose lets lane t store output column t, so loads and
+// stores are coalesced across the lanes of a row.  128 x 128 windows run on
+// the lane pairs of the 128 x 128 correlation kernel (corr_fft128.hip).
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -97,6 +98,7 @@ hipError_t launch_fft_transform(const double* in, double* out, int win, size_t n
     case 16: return launch_w<16>(in, out, n, forward, stream);
     case 32: return launch_w<32>(in, out, n, forward, stream);
     case 64: return launch_w<64>(in, out, n, forward, stream);
+    case 128: return launch_fft_transform128(in, out, n, forward, stream);
     default: return hipErrorInvalidValue;
     }
 }

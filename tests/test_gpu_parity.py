@@ -261,8 +261,10 @@ def test_window_128_dtypes_limit_and_real(gpu_ctx, dtype):
 
 
 def test_window_128_rejected_where_unsupported(gpu_ctx):
+    """the adjoint and the direct path stop below 128 x 128 windows; 256 is
+    outside the engine's window range"""
     with pytest.raises(oa.OpivError) as e:
-        gpu_ctx.transform(np.zeros((128, 128)))
+        gpu_ctx.transform(np.zeros((256, 256)))
     assert e.value.code == 6
     plan = oa.Plan(gpu_ctx, 512, 512, 128, dtype=np.float32)
     a = np.zeros((512, 512), np.float32)
@@ -347,7 +349,7 @@ def test_errors(gpu_ctx):
     assert rc == 3 and "differs" in oa.load().opiv_last_error().decode()
 
 
-@pytest.mark.parametrize("win", [8, 16, 32, 64])
+@pytest.mark.parametrize("win", [8, 16, 32, 64, 128])
 def test_transform_matches_oracle(gpu_ctx, win):
     """FFT::transform (fft.h:86-119) of stacked complex windows, both
     directions, against the FP64 oracle; plus the DFT of a single complex
