@@ -327,7 +327,8 @@ void corr_direct32_kernel(CorrParams p)
 
     // ---- planes to the column layout of the FFT tail: lane (slot, t) reads
     // column t of its windows' planes, one round at a time, the last round
-    // first, so that the slice still holds round 0 for the tail ----
+    // first, so that the slice still holds the first round's planes for the
+    // tail ----
     float s[ROUNDS][W];
     sfor<ROUNDS>([&](auto Rq) {
         constexpr int r = ROUNDS - 1 - decltype(Rq)::value;
@@ -356,13 +357,14 @@ void corr_direct32_kernel(CorrParams p)
     // unnormalised inverse FFT), an exact power of two that the tail restores
     float unscale[ROUNDS];
     sfor<ROUNDS>([&](auto R) { unscale[decltype(R)::value] = (float)(W * W); });
-    // the slice holds one round's planes (both groups): round 0 is there,
-    // round 1 goes back from the column layout when the tail asks for it
+    // the slice holds one round's planes (both groups): the first round's
+    // are there, the second's go back from the column layout when the tail
+    // asks for them
     float* pl = slice + slot * W * (W + 1);
     auto stage = [&](auto R) {
         constexpr int r = decltype(R)::value;
         if constexpr (r > 0) {
-            wave_sync();                                    // round 0's readers are done
+            wave_sync();                                    // the first round's readers are done
             sfor<W>([&](auto Hh) {
                 constexpr int h = decltype(Hh)::value;
                 pl[t * (W + 1) + h] = s[r][h];
