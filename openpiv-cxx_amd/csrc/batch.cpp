@@ -59,7 +59,9 @@ bool cache_lookup(const std::string& key, int cu, Shape& s)
         if (line.rfind(key + " ", 0) != 0) continue;
         std::istringstream r(line.substr(key.size() + 1));
         int nt = 0, bpc4 = 0;
-        if (r >> nt >> bpc4 && (nt == 64 || nt == 256) && bpc4 >= 0) {
+        // blocks per CU x 4 up to 256 (64 blocks per CU): a bound that keeps
+        // the grid arithmetic in range whatever the file holds
+        if (r >> nt >> bpc4 && (nt == 64 || nt == 256) && bpc4 >= 0 && bpc4 <= 256) {
             s.nt = nt;
             s.blocks = bpc4 * cu / 4;
             return true;
@@ -393,7 +395,7 @@ int opiv_batch_create(opiv_ctx* ctx, const opiv_batch_desc* desc, const opiv_pai
             b->single_blocks = 2 * cu;
         } else if (form_env && std::string(form_env) == "perpass") {
             b->single = false;
-        } else if (cache_lookup_raw(fk, v0, v1) && (v0 == 0 || (v0 == 1 && v1 > 0))) {
+        } else if (cache_lookup_raw(fk, v0, v1) && (v0 == 0 || (v0 == 1 && v1 > 0 && v1 <= 256))) {
             b->single = v0 == 1;
             b->single_blocks = v1 * cu / 4;
         } else if (may_time) {
