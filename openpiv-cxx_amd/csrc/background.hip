@@ -227,15 +227,211 @@ __global__ __launch_bounds__(256) void bg_scale_kernel(BgParams p)
     });
 }
 
+// ---- the same three passes on runs of V = 8 consecutive pixels per thread,
+// each run one (or, for 4- and 8-byte pixels, two or four) 16-byte load, each
+// output run one 16-byte store: a wave moves 1-4 KB per instruction where the
+// forms above move 128 B.  Every pixel's arithmetic is the same. ----
+constexpr int V = 8;
+constexpr int VSPAN = 256 * V;
+
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <int DT>
+__device__ __forceinline__ void load_run(const void* base, size_t idx, double (&v)[V])
+{
+    if constexpr (DT == 0) {
+        const u32x2 w = *(gptr<u32x2>)((gptr<uint8_t>)base + idx);
+        sfor<V>([&](auto E) {
+            constexpr int e = decltype(E)::value;
+            v[e] = (double)(((e < 4 ? w.x : w.y) >> (8 * (e & 3))) & 0xffu);
+        });
+    } else if constexpr (DT == 1) {
+        const u32x4 w = *(gptr<u32x4>)((gptr<uint16_t>)base + idx);
+        const uint32_t q[4] = {w.x, w.y, w.z, w.w};
+        sfor<V>([&](auto E) {
+            constexpr int e = decltype(E)::value;
+            v[e] = (double)((q[e / 2] >> (16 * (e & 1))) & 0xffffu);
+        });
+    } else if constexpr (DT == 2) {
+        const gptr<f32x4> f = (gptr<f32x4>)((gptr<float>)base + idx);
+        const f32x4 a = f[0], b = f[1];
+        const float q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        sfor<V>([&](auto E) { v[decltype(E)::value] = (double)q[decltype(E)::value]; });
+    } else {
+        const gptr<f64x2> f = (gptr<f64x2>)((gptr<double>)base + idx);
+        sfor<V / 2>([&](auto E) {
+            constexpr int e = decltype(E)::value;
+            const f64x2 d = f[e];
+            v[2 * e] = d.x;
+            v[2 * e + 1] = d.y;
+        });
+    }
+}
+
+__device__ __forceinline__ void load_mean(const double* mean, size_t idx, double (&m)[V])
+{
+    const gptr<f64x2> f = (gptr<f64x2>)((gptr<double>)mean + idx);
+    sfor<V / 2>([&](auto E) {
+        constexpr int e = decltype(E)::value;
+        const f64x2 d = f[e];
+        m[2 * e] = d.x;
+        m[2 * e + 1] = d.y;
+    });
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void bg_mean_vec_kernel(BgParams p)
+{
+    const int x = (blockIdx.x * 256 + (int)threadIdx.x) * V;
+    const int y = blockIdx.y;
+    if (x >= p.w) return;
+    const size_t row = (size_t)y * p.pitch;
+    double acc[V];
+    sfor<V>([&](auto E) { acc[decltype(E)::value] = 0.0; });
+    constexpr int U = 2;                         // frames loaded ahead of their adds
+    for (int i = 1; i < p.n; i += U) {
+        double v[U][V];
+        sfor<U>([&](auto Uu) {
+            constexpr int u = decltype(Uu)::value;
+            load_run<DT>(p.frames[i + u < p.n ? i + u : i], row + x, v[u]);
+        });
+        sfor<U>([&](auto Uu) {
+            constexpr int u = decltype(Uu)::value;
+            if (i + u < p.n)
+                sfor<V>([&](auto E) {
+                    constexpr int e = decltype(E)::value;
+                    acc[e] = acc[e] + v[u][e];
+                });
+        });
+    }
+    const double div = (double)(p.n - 1);
+    double2* dst = reinterpret_cast<double2*>(p.mean + (size_t)y * p.w + x);
+    sfor<V / 2>([&](auto E) {
+        constexpr int e = decltype(E)::value;
+        dst[e] = make_double2(acc[2 * e] / div, acc[2 * e + 1] / div);
+    });
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void bg_range_vec_kernel(BgParams p)
+{
+    const int f0 = blockIdx.z * CH;
+    const int x = (blockIdx.x * 256 + (int)threadIdx.x) * V;
+    const bool inside = x < p.w;
+    const void* fr[CH];
+    sfor<CH>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        fr[j] = p.frames[f0 + j < p.n ? f0 + j : f0];
+    });
+    double mn[CH], mx[CH];
+    sfor<CH>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        mn[j] = __longlong_as_double(0x7ff0000000000000ll);     // +inf
+        mx[j] = -mn[j];
+    });
+    if (inside) {
+        for (int y = blockIdx.y; y < p.h; y += gridDim.y) {
+            const size_t row = (size_t)y * p.pitch;
+            double m[V];
+            load_mean(p.mean, (size_t)y * p.w + x, m);
+            sfor<CH>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                if (f0 + j < p.n) {
+                    double v[V];
+                    load_run<DT>(fr[j], row + x, v);
+                    sfor<V>([&](auto E) {
+                        constexpr int e = decltype(E)::value;
+                        const double d = v[e] - m[e];
+                        mn[j] = d < mn[j] ? d : mn[j];
+                        mx[j] = d > mx[j] ? d : mx[j];
+                    });
+                }
+            });
+        }
+    }
+    __shared__ double smn[4][CH], smx[4][CH];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    sfor<CH>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        double a = mn[j], b = mx[j];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double oa = __shfl_xor(a, off, 64), ob = __shfl_xor(b, off, 64);
+            a = oa < a ? oa : a;
+            b = ob > b ? ob : b;
+        }
+        if (lane == 0) {
+            smn[wave][j] = a;
+            smx[wave][j] = b;
+        }
+    });
+    __syncthreads();
+    if (threadIdx.x < CH && f0 + (int)threadIdx.x < p.n) {
+        const int j = threadIdx.x;
+        double a = smn[0][j], b = smx[0][j];
+        for (int w = 1; w < 4; ++w) {
+            a = smn[w][j] < a ? smn[w][j] : a;
+            b = smx[w][j] > b ? smx[w][j] : b;
+        }
+        if (a <= b) {                            // the block saw pixels
+            atomicMin(&p.range[2 * (f0 + j)], okey(a));
+            atomicMax(&p.range[2 * (f0 + j) + 1], okey(b));
+        }
+    }
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void bg_scale_vec_kernel(BgParams p)
+{
+    const int x = (blockIdx.x * 256 + (int)threadIdx.x) * V;
+    const int y = blockIdx.y;
+    const int f0 = blockIdx.z * CH;
+    if (x >= p.w) return;
+    const size_t row = (size_t)y * p.pitch;
+    double m[V];
+    load_mean(p.mean, (size_t)y * p.w + x, m);
+    sfor<CH>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const int f = f0 + j;
+        if (f < p.n) {
+            const double lo = ovalue(p.range[2 * f]), hi = ovalue(p.range[2 * f + 1]);
+            double range = hi - lo;
+            if (hi == lo) range = 1.0;
+            const double rinv = 1.0 / range;
+            double v[V];
+            load_run<DT>(p.frames[f], row + x, v);
+            uint32_t o[V / 2];
+            sfor<V / 2>([&](auto E) {
+                constexpr int e = decltype(E)::value;
+                const uint32_t lo16 = scale16(v[2 * e] - m[2 * e] - lo, range, rinv);
+                const uint32_t hi16 = scale16(v[2 * e + 1] - m[2 * e + 1] - lo, range, rinv);
+                o[e] = lo16 | (hi16 << 16);
+            });
+            *reinterpret_cast<uint4*>(p.out[f] + (size_t)y * p.w + x) = make_uint4(o[0], o[1], o[2], o[3]);
+        }
+    });
+}
+
 template <int DT>
 static hipError_t launch_dt(const BgParams& p, hipStream_t stream)
 {
-    const unsigned gx = (unsigned)((p.w + SPAN - 1) / SPAN), gy = (unsigned)p.h;
+    const unsigned gx = (unsigned)((p.w + (p.vec ? VSPAN : SPAN) - 1) / (p.vec ? VSPAN : SPAN)), gy = (unsigned)p.h;
     const unsigned chunks = (unsigned)((p.n + CH - 1) / CH);
-    hipLaunchKernelGGL((bg_mean_kernel<DT>), dim3(gx, gy), dim3(256), 0, stream, p);
-    hipLaunchKernelGGL(bg_range_init_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, stream, p.range, p.n);
     // about 8 rows of pixels per thread in the range pass
     const unsigned ry = gy < 8 ? gy : (gy + 7) / 8;
+    if (p.vec) {
+        hipLaunchKernelGGL((bg_mean_vec_kernel<DT>), dim3(gx, gy), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(bg_range_init_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, stream, p.range,
+                           p.n);
+        hipLaunchKernelGGL((bg_range_vec_kernel<DT>), dim3(gx, ry, chunks), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL((bg_scale_vec_kernel<DT>), dim3(gx, gy, chunks), dim3(256), 0, stream, p);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((bg_mean_kernel<DT>), dim3(gx, gy), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(bg_range_init_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, stream, p.range, p.n);
     hipLaunchKernelGGL((bg_range_kernel<DT>), dim3(gx, ry, chunks), dim3(256), 0, stream, p);
     hipLaunchKernelGGL((bg_scale_kernel<DT>), dim3(gx, gy, chunks), dim3(256), 0, stream, p);
     return hipGetLastError();

@@ -17,6 +17,8 @@ struct BgParams {
     size_t pitch;                  // frame elements per row
     double* mean;                  // w * h, row-major, device
     unsigned long long* range;     // 2 per frame: ordered keys of min and max
+    bool vec;                      // runs of 8 pixels as single loads and stores: w and pitch
+                                   // multiples of 8, every frame, output and mean 16-byte aligned
 };
 
 // mean over frames 1..n-1, per-frame range of frame - mean, 16-bit scaling

@@ -571,6 +571,13 @@ int opiv_average_subtract(opiv_ctx* ctx, const void* const* frames, size_t n, ui
     p.pitch = pitch;
     p.mean = dmean;
     p.range = static_cast<unsigned long long*>(ctx->bg_range.p);
+    // the run-of-8 kernels: every row start of every frame, output and mean
+    // on a 16-byte boundary (8-byte for 8-bit frames, whose runs are 8 bytes)
+    const size_t falign = esz == 1 ? 8 : 16;
+    bool vec = img_w % 8 == 0 && pitch % 8 == 0 && ((uintptr_t)dmean & 15) == 0;
+    for (size_t i = 0; i < n && vec; ++i)
+        vec = ((uintptr_t)dframes[i] % falign) == 0 && ((uintptr_t)douts[i] & 15) == 0;
+    p.vec = vec;
     HIP_TRY(opiv::launch_average_subtract(p, dtype, ctx->stream));
     for (size_t i = 0; i < n; ++i)
         if (host_out[i]) HIP_TRY(hipMemcpyAsync(out[i], douts[i], obytes, hipMemcpyDeviceToHost, ctx->stream));

@@ -40,6 +40,32 @@ def test_average_subtract_matches_oracle(gpu_ctx, dtype):
     assert np.array_equal(out, rout)
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32, np.float64])
+def test_average_subtract_runs_of_8_match_oracle(gpu_ctx, dtype):
+    """17 frames of 256 x 200 pixels (widths and pitches that are multiples
+    of 8 take the run-of-8 kernels), host frames and device frames with a
+    wider pitch"""
+    rng = np.random.default_rng(23)
+    hi = {np.uint8: 255, np.uint16: 65535}.get(dtype, 1.0)
+    n, h, w, pitch = 17, 200, 256, 264
+    full = (rng.random((n, h, pitch)) * hi).astype(dtype)
+    if dtype in (np.float32, np.float64):
+        full = full - 0.25
+    frames = np.ascontiguousarray(full[:, :, :w])
+    rmean, rout = orc.average_subtract(frames)
+    out, mean = gpu_ctx.average_subtract(frames, return_mean=True)
+    assert np.array_equal(mean, rmean)
+    assert np.array_equal(out, rout)
+    dframes = [oa.DeviceBuffer.from_array(full[i]) for i in range(n)]
+    douts = [oa.DeviceBuffer(h * w * 2) for _ in range(n)]
+    dmean = oa.DeviceBuffer(h * w * 8)
+    gpu_ctx.average_subtract_device([d.ptr for d in dframes], [d.ptr for d in douts], w, h, dtype,
+                                    pitch=pitch, mean_ptr=dmean.ptr)
+    assert np.array_equal(dmean.download(np.zeros((h, w))), rmean)
+    for i in range(n):
+        assert np.array_equal(douts[i].download(np.zeros((h, w), np.uint16)), rout[i])
+
+
 def test_average_subtract_device_pointers_and_pitch(gpu_ctx):
     """device frames with pitch > width, device outputs and mean"""
     rng = np.random.default_rng(3)
