@@ -37,17 +37,13 @@ __device__ __forceinline__ float from_right(float v)
 {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
-__device__ __forceinline__ float from_left(float v)
-{
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
-}
 
 // max(value of lane - 1, value of lane + 1, up, down): the left neighbour
 // enters through DPP as an operand of the max itself.  Written out because a
 // max of a DPP result in compiled code first canonicalises it (one more
 // instruction).  s_nop 1 gives the two wait states a DPP read needs after a
 // VALU write of its source.  Group-edge lanes take a neighbour group's value
-// or zero, as with from_left / from_right.
+// or zero, as with from_right.
 __device__ __forceinline__ float neighbour_max(float c, float up, float down)
 {
     const float r = from_right(c);
@@ -235,30 +231,53 @@ __device__ __forceinline__ int group_min(int v)
 // t: strict 4-neighbour maxima on rows [hlo, hhi] and columns [wlo, whi]
 // (image_utils.impl.h:48-64), reduced to the largest value (and its scan
 // index h*W + t; equal values keep scan order) and the second largest value
-// and a scan index of it over the W lanes of the window.
-template <int W>
-__device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, int hhi, bool col_ok,
+// and a scan index of it over the W lanes of the window.  own(h) reads the
+// lane's plane value at row h from LDS.  Rows go in pairs: of two vertically
+// adjacent elements at most one is a strict maximum (each must exceed the
+// other), so each pair offers one candidate to the top 2, and the pair's
+// number is all the scan keeps of its position.
+template <int W, class Own>
+__device__ __forceinline__ void scan_top2(const float (&s)[W], int t, int hlo, int hhi, bool col_ok, Own&& own,
                                           float& v1, int& i1, float& v2, int& i2)
 {
     v1 = -INFINITY;
     v2 = -INFINITY;
-    int h1 = 0x7fff;
-    sfor<W - 3>([&](auto Hm) {
-        constexpr int h = decltype(Hm)::value + 1;
-        const float c = s[h];
-        // strict maximum over the 4 neighbours; the region test is data
-        // flow, so mc stays a select of canonical values
-        const bool ismax = c > neighbour_max(c, s[h - 1], s[h + 1]);
-        const bool inreg = col_ok & (h >= hlo) & (h <= hhi);
-        const float mc = (ismax & inreg) ? c : -INFINITY;
-        const bool gt = mc > v1;                  // rows arrive in scan order: ties keep the first
+    int p1 = 0;
+    // candidate mc of pair P, in scan order: ties keep the first
+    auto offer = [&](float mc, int P) {
+        const bool gt = mc > v1;
         v2 = __builtin_amdgcn_fmed3f(v2, mc, v1); // = max(v2, min(mc, v1)), as v2 <= v1
-        h1 = gt ? h : h1;                         // h is an inline constant of the select
+        p1 = gt ? P : p1;                         // P is an inline constant of the select
         v1 = gt ? mc : v1;                        // a select is cheaper than a max here
+    };
+    // rows 1 .. W-3; the region's rows are wave-uniform tests, its columns
+    // (col_ok) are applied once after the scan.  The region test is data
+    // flow, so the candidate stays a select of canonical values.
+    constexpr int NR = W - 3;
+    sfor<NR / 2>([&](auto Pp) {
+        constexpr int P = decltype(Pp)::value;
+        constexpr int h = 2 * P + 1;
+        const float c0 = s[h], c1 = s[h + 1];
+        const bool m0 = (c0 > neighbour_max(c0, s[h - 1], c1)) & (h >= hlo) & (h <= hhi);
+        const bool m1 = (c1 > neighbour_max(c1, c0, s[h + 2])) & (h + 1 >= hlo) & (h + 1 <= hhi);
+        offer(m0 ? c0 : (m1 ? c1 : -INFINITY), P);
     });
-    // keeps the multiply out of the selects above (folded in, it turns every
-    // h into a non-inline constant held in a register)
-    asm("" : "+v"(h1));
+    if constexpr (NR % 2 == 1) {
+        constexpr int h = NR;
+        const float c = s[h];
+        const bool m = (c > neighbour_max(c, s[h - 1], s[h + 1])) & (h >= hlo) & (h <= hhi);
+        offer(m ? c : -INFINITY, NR / 2);
+    }
+    // keeps the arithmetic below out of the selects above (folded in, it
+    // turns every pair number into a non-inline constant held in a register)
+    asm("" : "+v"(p1));
+    v1 = col_ok ? v1 : -INFINITY;
+    v2 = col_ok ? v2 : -INFINITY;
+    // the best's row within its pair: the first row holds it exactly when
+    // its value is the best (the pair's other row then differs from it);
+    // a trailing single row is pair NR / 2's first
+    const int hp = 2 * p1 + 1;
+    const int h1 = own(hp) == v1 ? hp : hp + 1;
     const int li = h1 * W + t;                    // this lane's best as a scan index
     // top 2 of the group: the largest value; among the lanes holding it the
     // smallest index (scan order; indices of different lanes differ); then
@@ -330,18 +349,20 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
         int px[ROUNDS], py[ROUNDS];
         int px2[ROUNDS], py2[ROUNDS];          // the second peak (1, 1 when unknown)
         bool two[ROUNDS], at2[ROUNDS];
-        sfor<ROUNDS>([&](auto R) {
+        // the scan reads one value of the staged plane (its best's row)
+        auto scan = [&](auto R) {
             constexpr int r = decltype(R)::value;
             int i1;
             int i2;
-            scan_top2<W>(s[r], t, hlo, hhi, col_ok, v1[r], i1, v2[r], i2);
+            scan_top2<W>(s[r], t, hlo, hhi, col_ok, [&](int h) { return fetch(r, h, t); }, v1[r], i1, v2[r], i2);
             two[r] = v2[r] > -INFINITY;
             px[r] = two[r] ? (i1 & (W - 1)) : 1;
             py[r] = two[r] ? (i1 / W) : 1;
             at2[r] = two[r] & (i2 < W * W);
             px2[r] = at2[r] ? (i2 & (W - 1)) : 1;
             py2[r] = at2[r] ? (i2 / W) : 1;
-        });
+        };
+        if constexpr (!SEQ) sfor<ROUNDS>(scan);
         // stencil value j of round r: (l, c, r, up, down) = plane positions
         // (py, px - 1), (py, px), (py, px + 1), (py - 1, px), (py + 1, px)
         auto stencil = [&](int r, int j) -> float {
@@ -379,6 +400,7 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
             sfor<ROUNDS>([&](auto R) {
                 constexpr int r = decltype(R)::value;
                 stage(R);
+                scan(R);
                 auto at = [&](int j) {
                     const int cx = px[r] + (j == 0 ? -1 : (j == 2 ? 1 : 0));
                     const int cy = py[r] + (j == 3 ? -1 : (j == 4 ? 1 : 0));

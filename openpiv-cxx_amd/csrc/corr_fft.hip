@@ -380,13 +380,17 @@ __device__ __forceinline__ void corr_task(const CorrParams& p, int task, LdsT<W>
     });
 }
 
-// occupancy floor per window size: the most waves whose VGPR budget keeps
-// the register-resident lines of every instance (plane output, f64 frames,
-// batch with per-lane frame pointers and a task loop) without spilling.  The
-// hot instances need fewer registers and run more waves: 74 VGPRs (6 waves)
-// at 16 x 16, 124 (4 waves) at 32 x 32.
-template <int W, bool LOOP>
-constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (LOOP ? 4 : 5) : W == 32 ? 3 : 2; }
+// occupancy floor: the most waves whose VGPR budget keeps the
+// register-resident lines of the instance without spilling.  The hot
+// instances (result records, frames of up to 32 bits, one task per wave,
+// 256-thread blocks) run 6 waves per SIMD at 16 x 16 (80 VGPRs) and 4 at
+// 32 x 32 (128); plane output, f64 frames, one-wave batch blocks and the
+// task loop need more registers and run fewer.
+template <int W, bool LOOP, bool HOT>
+constexpr int waves_per_simd()
+{
+    return W <= 8 ? 4 : W == 16 ? (HOT ? 6 : LOOP ? 4 : 5) : W == 32 ? (HOT ? 4 : 3) : 2;
+}
 
 // Launch modes: one pair, one task per wave; a batch (each window's pair
 // from the device table) with one task per wave; a batch whose waves loop
@@ -396,7 +400,9 @@ constexpr int waves_per_simd() { return W <= 8 ? 4 : W == 16 ? (LOOP ? 4 : 5) : 
 enum LaunchMode { kOnePair = 0, kBatch = 1, kBatchLoop = 2 };
 
 template <int W, int DT, bool PLANES, int ROUNDS, int NT, int MODE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves_per_simd<W, MODE == kBatchLoop>())))
+__global__ __launch_bounds__(NT) __attribute__((
+    amdgpu_waves_per_eu(waves_per_simd<W, MODE == kBatchLoop,
+                                       !PLANES && DT != 3 && (MODE == kOnePair || (MODE == kBatch && NT == 256))>())))
 void corr_fft_kernel(CorrParams p)
 {
     constexpr int WPW = 64 / W;
