@@ -9,7 +9,8 @@ overlap): 65025 windows -> 65025 vectors per step.
 Timing: W untimed warm-up steps, then K launches bracketed by HIP events on
 the engine's own stream (opiv_ctx_stream) with device synchronisation on
 both sides.  Accuracy: a bounded sample of windows is compared with the FP64
-oracle and the max displacement error is reported.  The CPU baseline is the
+oracle: max and RMS displacement error, integer-peak mismatches outside the
+flagged windows, and the count of each flag.  The CPU baseline is the
 reference's own FP64 path (oracle/_ref/ref_piv, the reference's headers and
 thread pool) timed on this host's cores over the same frame pair, or the
 oracle restatement when the reference harness is absent.
@@ -201,9 +202,15 @@ def accuracy(res, a, b, win, step, sample=4096, seed=0):
     tie = (got["flags"] & (oa.FLAG_NEAR_TIE | oa.FLAG_EXACT_TIE | oa.FLAG_FEWER_THAN_2_PEAKS)) != 0
     ok = ~tie & np.isfinite(ref["u"]) & np.isfinite(ref["v"])
     mism = int(((got["px"] != ref["px"]) | (got["py"] != ref["py"]))[ok].sum())
-    err = float(max(np.abs(got["u"][ok] - ref["u"][ok]).max(), np.abs(got["v"][ok] - ref["v"][ok]).max()))
-    return {"max_err_px": err, "checked_windows": int(len(ref)), "peak_mismatches": mism,
-            "near_ties": int(tie.sum())}
+    du, dv = got["u"][ok] - ref["u"][ok], got["v"][ok] - ref["v"][ok]
+    err = float(max(np.abs(du).max(), np.abs(dv).max()))
+    rms = float(np.sqrt(np.mean(np.concatenate([du, dv]) ** 2)))
+    counts = {name: int(((got["flags"] & bit) != 0).sum())
+              for name, bit in (("near_tie", oa.FLAG_NEAR_TIE), ("exact_tie", oa.FLAG_EXACT_TIE),
+                                ("fewer_than_2_peaks", oa.FLAG_FEWER_THAN_2_PEAKS),
+                                ("nonfinite_fit", oa.FLAG_NONFINITE_FIT))}
+    return {"max_err_px": err, "rms_err_px": rms, "checked_windows": int(len(ref)), "peak_mismatches": mism,
+            "near_ties": int(tie.sum()), "flags": counts}
 
 
 def copy_calibration(mib: int, reps: int):
