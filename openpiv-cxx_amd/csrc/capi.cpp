@@ -177,6 +177,7 @@ static opiv::CorrParams plan_params(const opiv_plan* plan, const void* a, const 
     p.step_x = (int32_t)plan->off_x;
     p.step_y = (int32_t)plan->off_y;
     p.nx = (int32_t)plan->nx;
+    p.nx_mul = opiv::fastdiv_mul(plan->nx);
     p.first = 0;
     p.n_windows = (int32_t)(plan->nx * plan->ny);
     p.img_h = plan->desc.img_h;
@@ -290,6 +291,7 @@ static int correlate_stacked(opiv_ctx* ctx, uint32_t win, size_t n, int variant,
     p.step_x = (int32_t)win;
     p.step_y = (int32_t)win;
     p.nx = 1;
+    p.nx_mul = opiv::fastdiv_mul(1);
     p.first = 0;
     p.n_windows = (int32_t)n;
     p.img_h = (uint32_t)(n * win);

@@ -160,6 +160,14 @@ struct WinGeom {
 template <class T>
 using gptr_t = const __attribute__((address_space(1))) T*;
 
+// g / nx for a grid width nx >= 1 and its multiplier mul = fastdiv_mul(nx):
+// one multiply-high and shifts instead of an integer division
+__device__ __forceinline__ int div_grid_width(int g, int nx, uint32_t mul)
+{
+    const uint32_t l = 32u - (uint32_t)__clz(nx - 1);   // ceil(log2 nx); 0 for nx = 1
+    return (int)fastdiv((uint32_t)g, l, mul);
+}
+
 // Geometry of window `local` of a launch: the single pair of the params, or
 // (batch) the pair whose window range [first, first + nx * ny) holds it.
 template <bool BATCH>
@@ -170,7 +178,7 @@ __device__ __forceinline__ WinGeom window_geom(const CorrParams& p, int local)
     if constexpr (!BATCH) {
         const int l = g.valid ? local : p.n_windows - 1;
         const int gid = p.first + l;
-        const int ix = gid % p.nx, iy = gid / p.nx;
+        const int iy = div_grid_width(gid, p.nx, p.nx_mul), ix = gid - iy * p.nx;
         g.a = p.a;
         g.b = p.b;
         g.pitch = p.pitch;
@@ -201,7 +209,7 @@ __device__ __forceinline__ WinGeom window_geom(const CorrParams& p, int local)
             g.valid = false;
             gid = gid < 0 ? 0 : n - 1;
         }
-        const int ix = gid % bp.nx, iy = gid / bp.nx;
+        const int iy = div_grid_width(gid, bp.nx, bp.nx_mul), ix = gid - iy * bp.nx;
         g.a = bp.a;
         g.b = bp.b;
         g.pitch = bp.pitch;

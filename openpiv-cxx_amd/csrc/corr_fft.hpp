@@ -21,6 +21,7 @@ struct CorrParams {
     int32_t x_start, y_start;
     int32_t step_x, step_y;
     int32_t nx;
+    uint32_t nx_mul;         // fastdiv_mul(nx)
     int32_t first;
     int32_t n_windows;
     uint32_t img_h;          // for the y flip of main.cpp:230

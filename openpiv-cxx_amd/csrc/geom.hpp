@@ -34,8 +34,38 @@ struct BatchPair {
     uint32_t img_h;
     int32_t first;
     int32_t out_first;
-    int32_t pad_;
+    uint32_t nx_mul;         // fastdiv_mul(nx)
 };
+
+// Division by a grid width without a divide: the round-up method of
+// Granlund and Montgomery (PLDI 1994, fig. 4.1).  For 1 <= d < 2^32 with
+// l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1, every 32-bit n has
+// n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0), t = (m n) >> 32.
+// constexpr, so the kernels use the same code (t is one multiply-high).
+constexpr uint32_t ceil_log2_u32(uint32_t d)
+{
+    uint32_t l = 0;
+    while (l < 32 && (uint64_t(1) << l) < d) ++l;
+    return l;
+}
+constexpr uint32_t fastdiv_mul(uint32_t d)
+{
+    return (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << ceil_log2_u32(d)) - d)) / d + 1);
+}
+constexpr uint32_t fastdiv(uint32_t n, uint32_t l, uint32_t m)
+{
+    const uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
+    return (t + ((n - t) >> (l < 1 ? l : 1))) >> (l > 0 ? l - 1 : 0);
+}
+static_assert(fastdiv(0, ceil_log2_u32(1), fastdiv_mul(1)) == 0 &&
+              fastdiv(4294967295u, ceil_log2_u32(1), fastdiv_mul(1)) == 4294967295u &&
+              fastdiv(65024, ceil_log2_u32(255), fastdiv_mul(255)) == 254 &&
+              fastdiv(65025, ceil_log2_u32(255), fastdiv_mul(255)) == 255 &&
+              fastdiv(4294967295u, ceil_log2_u32(7), fastdiv_mul(7)) == 613566756u &&
+              fastdiv(4294967295u, ceil_log2_u32(2147483649u), fastdiv_mul(2147483649u)) == 1 &&
+              fastdiv(1023, ceil_log2_u32(1024), fastdiv_mul(1024)) == 0 &&
+              fastdiv(1024, ceil_log2_u32(1024), fastdiv_mul(1024)) == 1,
+              "round-up division by a multiplier");
 
 // windows per wave task of the kernel for window size win and pixel type
 // dtype (0 u8, 1 u16, 2 f32, 3 f64): two rounds of 64 / win lane groups, one

@@ -138,6 +138,7 @@ int check_batch_pair(const opiv_pair_desc& pr, size_t i, uint32_t pass, uint32_t
     e->nx = (int32_t)nx;
     e->ny = (int32_t)ny;
     e->img_h = pr.img_h;
+    e->nx_mul = fastdiv_mul(nx);
     return OPIV_OK;
 }
 

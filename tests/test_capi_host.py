@@ -109,3 +109,13 @@ def test_batch_entry_points_reject_null_handles():
         assert rc == 1, (what, rc)
         assert oa.last_error(), what
     assert lib.opiv_batch_destroy(None) == 0
+
+
+def test_grid_width_division_matches_divide(tmp_path):
+    """The kernels find a window's grid row with geom.hpp's multiplier
+    division; checked against the divide instruction on the host."""
+    exe = tmp_path / "fastdiv_check"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", str(ROOT / "openpiv-cxx_amd" / "csrc"),
+                    str(ROOT / "tests" / "cpp" / "fastdiv_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
