@@ -48,9 +48,10 @@ constexpr uint32_t ceil_log2_u32(uint32_t d)
     while (l < 32 && (uint64_t(1) << l) < d) ++l;
     return l;
 }
+// (d = 0 gives 0: no grid has zero columns, and the host never divides by it)
 constexpr uint32_t fastdiv_mul(uint32_t d)
 {
-    return (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << ceil_log2_u32(d)) - d)) / d + 1);
+    return d == 0 ? 0u : (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << ceil_log2_u32(d)) - d)) / d + 1);
 }
 constexpr uint32_t fastdiv(uint32_t n, uint32_t l, uint32_t m)
 {
