@@ -185,3 +185,52 @@ def test_vjp_async_matches_sync(gpu_ctx):
     with pytest.raises(oa.OpivError) as e:
         oa._check(lib.opiv_vjp_pair_async(plan.handle, da.ptr, db.ptr, None, dcot.ptr, dga.ptr, dgb.ptr))
     assert e.value.code == 1
+
+
+def _random_adjoint_case(seed):
+    rng = np.random.default_rng(2000 + seed)
+    win = int(rng.choice((8, 16, 32, 64)))
+    dtype = (np.uint8, np.uint16, np.float32, np.float64)[seed % 4]
+    w = int(rng.integers(win, win + 300))
+    h = int(rng.integers(win, win + 260))
+    pitch = w if rng.random() < 0.5 else w + int(rng.integers(1, 24))
+    step = int(rng.integers(max(1, win // 4), win + win // 2 + 1)) if rng.random() < 0.5 else win // 2
+    limit = bool(rng.random() < 0.3)
+    shift = tuple(float(v) for v in rng.uniform(-win / 6, win / 6, size=2))
+    return win, dtype, w, h, pitch, step, limit, shift
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_vjp_and_variance_random_plans(gpu_ctx, seed):
+    """Seeded random plans (window 8-64, every pixel type, uneven frames with
+    junk in the pitch padding, steps from W/4 to 1.5 W, limit-search on some):
+    whole-frame gradients of a random cotangent against the FP64 CPU adjoint,
+    zero in the padding, and the variances against the FP64 variance"""
+    win, dtype, w, h, pitch, step, limit, shift = _random_adjoint_case(seed)
+    say(f"adjoint case {seed}: w{win} {np.dtype(dtype).name} {w}x{h} pitch {pitch} step {step} limit {limit}")
+    i0, noise = {np.uint8: ((30.0, 250.0), 3.0), np.uint16: ((1000.0, 60000.0), 50.0)}.get(dtype, ((100.0, 255.0), 2.0))
+    a, b = oa.synth_pair(w, h, seed=90 + seed, i0=i0, noise=noise, params=shift, dtype=dtype)
+    rng = np.random.default_rng(seed)
+    pa = np.zeros((h, pitch), dtype)
+    pb = np.zeros((h, pitch), dtype)
+    pa[:, :w], pb[:, :w] = a, b
+    pa[:, w:] = rng.uniform(0, 200, size=(h, pitch - w)).astype(dtype)
+    pb[:, w:] = rng.uniform(0, 200, size=(h, pitch - w)).astype(dtype)
+    plan = oa.Plan(gpu_ctx, w, h, win, step=(step, step), dtype=dtype, pitch=pitch if pitch != w else 0,
+                   limit_search=limit)
+    fwd = plan.run(pa, pb)
+    skip = (fwd["flags"] & (oa.FLAG_NEAR_TIE | oa.FLAG_EXACT_TIE | oa.FLAG_NONFINITE_FIT)) != 0
+    cot = rng.normal(size=(plan.count, 2))
+    cot[skip] = 0
+    ga, gb = plan.vjp(pa, pb, cot, fwd)
+    ra, rb = orc.vjp_pair(a, b, win, step, step, cot, limit_search=limit)
+    scale = max(np.abs(ra).max(), np.abs(rb).max())
+    if scale > 0:
+        for got, ref in ((ga, ra), (gb, rb)):
+            err = np.abs(got[:, :w] - ref).max() / scale
+            assert err <= GRAD_TOL, (seed, err)
+    assert not ga[:, w:].any() and not gb[:, w:].any(), seed
+    var = plan.variance(pa, pb, 2.0, fwd)
+    ref = orc.variance_pair(a, b, win, step, step, 2.0, limit_search=limit)
+    ok = ~skip & ((fwd["flags"] & oa.FLAG_FEWER_THAN_2_PEAKS) == 0)
+    assert np.allclose(var[ok], ref[ok], rtol=GRAD_TOL, atol=0), (seed, np.abs(var[ok] / ref[ok] - 1).max())
