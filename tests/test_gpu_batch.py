@@ -241,3 +241,40 @@ def test_128_pass_stays_per_pass(gpu_ctx, batch_setup, monkeypatch):
     sub = [p for p, (a, _) in zip(pairs, frames) if min(a.shape) >= 128]
     monkeypatch.setenv("OPIV_BATCH_FORM", "single")
     assert oa.Batch(gpu_ctx, sub, (128, 32)).form() == (False, 0)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_batches_equal_single_pair_path(gpu_ctx, monkeypatch, seed):
+    """Seeded random batches: 1-6 pairs of uneven sizes, 1-4 passes of any
+    window size 8-128 with their own step fractions, either variant,
+    limit-search on some, the launch form forced or tuned.  Every window of
+    every pass and pair is bit-identical to the single-pair launch of the
+    same plan."""
+    rng = np.random.default_rng(3000 + seed)
+    dtype = (np.uint8, np.uint16, np.float32, np.float64)[seed % 4]
+    npasses = int(rng.integers(1, 5))
+    wins = tuple(int(w) for w in rng.choice((8, 16, 32, 64, 128), size=npasses, p=(0.15, 0.3, 0.3, 0.15, 0.1)))
+    overlaps = [float(o) for o in rng.choice((0.25, 0.5, 0.75, 1.0), size=npasses)]
+    variant = (oa.CIRCULAR_FFT_FP32, oa.REAL_COMPAT)[int(rng.integers(2))]
+    limit = bool(rng.random() < 0.3)
+    form = ("perpass", "single", None)[seed % 3]
+    lo = max(wins)
+    sizes = [(int(rng.integers(lo, lo + 500)), int(rng.integers(lo, lo + 400))) for _ in range(int(rng.integers(1, 7)))]
+    say(f"random batch {seed}: {np.dtype(dtype).name} wins {wins} overlaps {overlaps} variant {variant} "
+        f"limit {limit} form {form} sizes {sizes}")
+    frames = [oa.synth_pair(w, h, seed=400 + seed * 8 + k, params=tuple(rng.uniform(-3, 3, size=2)), dtype=dtype)
+              for k, (w, h) in enumerate(sizes)]
+    bufs = [(oa.DeviceBuffer.from_array(a), oa.DeviceBuffer.from_array(b)) for a, b in frames]
+    pairs = [(da.ptr, db.ptr, a.shape[1], a.shape[0]) for (a, _), (da, db) in zip(frames, bufs)]
+    if form:
+        monkeypatch.setenv("OPIV_BATCH_FORM", form)
+    else:
+        monkeypatch.delenv("OPIV_BATCH_FORM", raising=False)
+    batch = oa.Batch(gpu_ctx, pairs, wins, overlaps, dtype=dtype, variant=variant, limit_search=limit)
+    _, res = run_batch(batch)
+    for p, (win, ov) in enumerate(zip(wins, overlaps)):
+        for i, (a, b) in enumerate(frames):
+            plan = oa.Plan(gpu_ctx, a.shape[1], a.shape[0], win, overlap=ov, dtype=dtype, variant=variant,
+                           limit_search=limit)
+            one = plan.run(bufs[i][0].ptr, bufs[i][1].ptr)
+            assert batch.block(res, p, i).tobytes() == one.tobytes(), (seed, p, win, i)
