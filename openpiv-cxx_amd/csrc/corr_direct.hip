@@ -104,29 +104,43 @@ __device__ __forceinline__ float from_quad_next(float v)
 
 } // namespace
 
+// 8-bit frames without plane output take the tail one round (four windows)
+// at a time: the wave's LDS then holds one window's operands or four planes
+// (4.3 KB instead of 8.7 KB), which lets six waves per SIMD fit instead of
+// four.  The kernel is latency-bound (the GEMMs, their staging and the tail
+// are short dependent chains), so the extra waves are what it needs.
+template <int DT, bool PLANES>
+constexpr bool by_rounds() { return DT == 0 && !PLANES; }
+
+template <int DT, bool PLANES>
+constexpr int direct16_waves() { return by_rounds<DT, PLANES>() ? 6 : DT <= 1 ? 4 : (DT == 2 ? 3 : 2); }
+
 // Wave task: 8 windows, (task * 2 + r) * 4 + slot for round r and slot --
 // the numbering and column layout of the 16 x 16 FFT kernel, so the shared
 // tail (finish_windows) turns the planes into results.  Each window in turn:
 // stage its bf16 terms in LDS, run the MFMA GEMM, and park the scaled,
 // quadrant-swapped plane in LDS; then lane (slot, t) reads column t of its
-// two windows' planes.
+// two windows' planes.  (8-bit frames: each round's four windows, then that
+// round's part of the tail.)
 template <int DT, bool PLANES>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4 : (DT == 2 ? 3 : 2)))) void corr_direct16_kernel(CorrParams p)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(direct16_waves<DT, PLANES>()))) void corr_direct16_kernel(CorrParams p)
 {
     constexpr int W = 16;
     constexpr int NP = nparts<DT>();
     constexpr int WAVES = 4;
     constexpr int ROUNDS = 2, WPW = 4, PER = ROUNDS * WPW;
+    constexpr bool RND = by_rounds<DT, PLANES>();
     // per wave: the operand staging of one window (rows of a; doubled rows of
-    // b, two copies), later overlaid by the 8 swapped planes
+    // b, two copies), later overlaid by the 8 swapped planes (RND: the 4
+    // planes of one round)
     // A rows padded to 12 dwords: the 8 rows of a b128 read group start in
     // distinct bank quads.  The odd-shift copy of B sits 16 dwords off the
     // even one, so that even and odd lanes of a read hit different banks.
     constexpr int APITCH = 24, BCOPY = 2 * W * W + 32;
     typedef uint16_t StageA[NP][W][APITCH];
     typedef uint16_t StageB[NP][2][BCOPY];
-    typedef float Planes[PER][W][W + 1];
-    constexpr int STAGE = (int)(sizeof(StageA) + sizeof(StageB));
+    typedef float Planes[RND ? WPW : PER][W][W + 1];
+    constexpr int STAGE = RND ? 0 : (int)(sizeof(StageA) + sizeof(StageB));
     constexpr int BYTES = STAGE > (int)sizeof(Planes) ? STAGE : (int)sizeof(Planes);
     __shared__ __attribute__((aligned(16))) unsigned char lds[WAVES][BYTES];
 
@@ -146,7 +160,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
     });
 
     StageA& lds_a = *reinterpret_cast<StageA*>(&lds[wave][0]);
-    StageB& lds_b = *reinterpret_cast<StageB*>(&lds[wave][sizeof(StageA)]);
+    StageB& lds_b = *reinterpret_cast<StageB*>(&lds[wave][RND ? 0 : sizeof(StageA)]);
     Planes& lds_p = *reinterpret_cast<Planes*>(&lds[wave][0]);
 
     const int i16 = lane & 15;                  // MFMA: row of A / column of B / column of D
@@ -332,6 +346,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
         return out;
     };
 
+    // D[sy][sx]: sx = i16, sy = 4 g + r.  The reference's plane is the
+    // un-normalised inverse FFT, W^2 times C (an exact power of two), and the
+    // quadrant swap puts C[sy][sx] at plane (sy + 8) % 16, (sx + 8) % 16.
+    const int wcol = (i16 + 8) & (W - 1);
+    // the sums are the correlation divided by W * W (the reference's
+    // unnormalised inverse FFT), an exact power of two that the tail restores
+    float unscale[ROUNDS];
+    sfor<ROUNDS>([&](auto R) { unscale[decltype(R)::value] = (float)(W * W); });
+
+    if constexpr (RND) {
+        // round r: its four windows' GEMMs, their planes to LDS, column t of
+        // this lane group's plane; the tail scans it and reads the stencils
+        // it needs before the next round overwrites the LDS
+        float s[ROUNDS][W];
+        auto run_round = [&](auto R) {
+            constexpr int r = decltype(R)::value;
+            floatx4 acc[WPW];
+            wave_sync();                        // the previous round's readers are done
+            sfor<WPW>([&](auto S) {
+                constexpr int i = r * WPW + decltype(S)::value;
+                const int sums = stage8(std::integral_constant<int, i>{});
+                wave_sync();
+                acc[decltype(S)::value] = gemm8(128 * sums + W * W * 128 * 128);
+                wave_sync();                    // staging buffers free for the next window
+            });
+            sfor<WPW>([&](auto S) {
+                constexpr int j = decltype(S)::value;
+                sfor<4>([&](auto Rr) {
+                    constexpr int q = decltype(Rr)::value;
+                    lds_p[j][(4 * g + q + 8) & (W - 1)][wcol] = acc[j][q];
+                });
+            });
+            wave_sync();
+            sfor<W>([&](auto Hh) {
+                constexpr int h = decltype(Hh)::value;
+                s[r][h] = lds_p[slot][h][t];
+            });
+        };
+        finish_windows<W, ROUNDS, false, true>(p, s, unscale, geo, active, lane, t, run_round,
+                                                [&](int, int row, int col) { return lds_p[slot][row][col]; });
+        return;
+    } else {
+
     sfor<PER>([&](auto I) {
         constexpr int i = decltype(I)::value;
         if constexpr (DT == 0) {
@@ -346,10 +403,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
         wave_sync();                            // staging buffers free for the next window
     });
 
-    // D[sy][sx]: sx = i16, sy = 4 g + r.  The reference's plane is the
-    // un-normalised inverse FFT, W^2 times C (an exact power of two), and the
-    // quadrant swap puts C[sy][sx] at plane (sy + 8) % 16, (sx + 8) % 16.
-    const int wcol = (i16 + 8) & (W - 1);
     sfor<PER>([&](auto I) {
         constexpr int i = decltype(I)::value;
         sfor<4>([&](auto R) {
@@ -368,13 +421,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DT <= 1 ? 4
             s[r][h] = lds_p[r * WPW + slot][h][t];
         });
     });
-    // the sums are the correlation divided by W * W (the reference's
-    // unnormalised inverse FFT), an exact power of two that the tail restores
-    float unscale[ROUNDS];
-    sfor<ROUNDS>([&](auto R) { unscale[decltype(R)::value] = (float)(W * W); });
     finish_windows<W, ROUNDS, PLANES, false>(p, s, unscale, geo, active, lane, t, [](auto) {}, [&](int r, int row, int col) {
         return lds_p[r * WPW + slot][row][col];
     });
+    }
 }
 
 template <int DT>
