@@ -60,10 +60,11 @@ int check_window(uint32_t win)
     return OPIV_OK;
 }
 
-// OPIV_AUTO: the faster path as measured on MI355X at 4096 x 4096: the direct
-// kernel for 8-bit frames at 16 x 16 (exact int8 MFMA, 74 us against the FFT
-// kernel's 83 us), the FFT kernel otherwise (16 x 16 u16: 90 against 198 us,
-// f32: 89 against 309 us).
+// OPIV_AUTO: the faster path as measured on MI355X at 4096 x 4096, step W/2:
+// the direct kernel for 8-bit frames at 16 x 16 (exact int8 MFMA, 65.7 us
+// against the FFT kernel's 74.1 us), the FFT kernel otherwise (16 x 16 u16:
+// 76 against 173 us, f32: 82 against 254 us; 32 x 32 u8: 79 against 87 us,
+// u16: 81 against 220 us).
 int32_t auto_variant(int win, int dtype)
 {
     return win == 16 && dtype == OPIV_U8 ? OPIV_DIRECT_BF16_SPLIT : OPIV_CIRCULAR_FFT_FP32;
