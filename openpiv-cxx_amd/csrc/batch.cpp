@@ -92,6 +92,9 @@ bool cache_lookup_raw(const std::string& key, int& v0, int& v1)
     return false;
 }
 
+// replace key's line of the cache with "key value": the file is rewritten
+// to a per-process temporary and renamed over the old one, so a reader never
+// sees a partial file
 void cache_store_line(const std::string& key, const std::string& value)
 {
     const std::string path = cache_path();
@@ -115,23 +118,7 @@ void cache_store_line(const std::string& key, const std::string& value)
 
 void cache_store(const std::string& key, int cu, const Shape& s)
 {
-    const std::string path = cache_path();
-    std::vector<std::string> keep;
-    {
-        std::ifstream is(path);
-        std::string line;
-        while (std::getline(is, line))
-            if (!line.empty() && line.rfind(key + " ", 0) != 0) keep.push_back(line);
-    }
-    for (size_t pos = path.find('/', 1); pos != std::string::npos; pos = path.find('/', pos + 1))
-        mkdir(path.substr(0, pos).c_str(), 0755);
-    const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
-    std::ofstream os(tmp);
-    for (const auto& l : keep) os << l << "\n";
-    os << key << " " << s.nt << " " << (cu ? s.blocks * 4 / cu : 0) << "\n";
-    os.close();
-    if (os) std::rename(tmp.c_str(), path.c_str());
-    else std::remove(tmp.c_str());
+    cache_store_line(key, std::to_string(s.nt) + " " + std::to_string(cu ? s.blocks * 4 / cu : 0));
 }
 
 } // namespace
