@@ -15,6 +15,7 @@
 #include "corr_fft.hpp"
 #include "xcd.hpp"
 #include "fft_reg.hpp"
+#include "fit_log.hpp"
 
 namespace opiv {
 
@@ -442,7 +443,7 @@ __device__ __forceinline__ void finish_windows(const CorrParams& p, const float 
         sfor<NP>([&](auto K) {
             constexpr int k = decltype(K)::value;
             const int L = t + k * W;
-            lg[k] = log((double)(L < NV ? sv[k] : 1.f));
+            lg[k] = ln_of_float(L < NV ? sv[k] : 1.f);
         });
 
         // ---- lane t < ROUNDS of the group fits window t ----

@@ -335,8 +335,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void co
             const float v1 = key_value(g1), v2 = key_value(g2);
             auto at = [&](int row, int col) { return plane[row * LP + col]; };
             const float l_ = at(py, px - 1), r_ = at(py, px + 1), u_ = at(py - 1, px), d_ = at(py + 1, px);
-            const double lnl = log((double)l_), lnc = log((double)v1), lnr = log((double)r_);
-            const double lnu = log((double)u_), lnd = log((double)d_);
+            const double lnl = ln_of_float(l_), lnc = ln_of_float(v1), lnr = ln_of_float(r_);
+            const double lnu = ln_of_float(u_), lnd = ln_of_float(d_);
             // image_utils.impl.h:86-98
             const double numx = lnl - lnr, denx = 2.0 * (lnl + lnr - 2.0 * lnc);
             const double numy = lnu - lnd, deny = 2.0 * (lnu + lnd - 2.0 * lnc);

@@ -111,6 +111,18 @@ def test_batch_entry_points_reject_null_handles():
     assert lib.opiv_batch_destroy(None) == 0
 
 
+def test_fit_logarithm_against_extended_precision(tmp_path):
+    """The Gaussian fit's FP64 logarithm of FP32 plane values (fit_log.hpp,
+    compiled into the kernels) against the x87 extended-precision logl:
+    every float of [1/8, 8), a stride over the whole positive range and the
+    special values; at most 1.5 ulp."""
+    exe = tmp_path / "fit_log_check"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", str(ROOT / "openpiv-cxx_amd" / "csrc"),
+                    str(ROOT / "tests" / "cpp" / "fit_log_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stdout
+
+
 def test_grid_width_division_matches_divide(tmp_path):
     """The kernels find a window's grid row with geom.hpp's multiplier
     division; checked against the divide instruction on the host."""
