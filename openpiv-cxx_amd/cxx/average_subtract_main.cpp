@@ -12,6 +12,7 @@
 // frames are widened to 16 bits when the set mixes them -- the same values).
 #include <cstdint>
 #include <cstdio>
+#include <exception>
 #include <fstream>
 #include <iostream>
 #include <string>
@@ -23,9 +24,15 @@ namespace {
 
 void log_error(const std::string& m) { std::cerr << m << "\n"; }
 
-} // namespace
+// main.cpp:123 logs "failed to load image: " and the loader's message; the
+// engine's decoder messages may carry that prefix already
+std::string load_error()
+{
+    const std::string prefix = "failed to load image: ", m = opiv_last_error();
+    return m.rfind(prefix, 0) == 0 ? m : prefix + m;
+}
 
-int main(int argc, char* argv[])
+int run(int argc, char* argv[])
 {
     std::vector<std::string> inputs;
     for (int i = 1; i < argc; ++i) {
@@ -53,7 +60,7 @@ int main(int argc, char* argv[])
     std::vector<opiv_image_info> info(inputs.size());
     for (size_t i = 0; i < inputs.size(); ++i) {
         if (opiv_image_probe(inputs[i].c_str(), &info[i]) != OPIV_OK) {
-            log_error(std::string("failed to load image: ") + opiv_last_error());
+            log_error(load_error());
             return 1;
         }
         if (info[i].width != info[0].width || info[i].height != info[0].height) {
@@ -79,14 +86,14 @@ int main(int argc, char* argv[])
                 dst = px8[i].data();
             }
             if (opiv_image_read(nullptr, inputs[i].c_str(), 0, dst, 0) != OPIV_OK) {
-                log_error(std::string("failed to load image: ") + opiv_last_error());
+                log_error(load_error());
                 return 1;
             }
             frames[i] = dst;
         } else {
             std::vector<uint8_t> tmp(n);
             if (opiv_image_read(nullptr, inputs[i].c_str(), 0, tmp.data(), 0) != OPIV_OK) {
-                log_error(std::string("failed to load image: ") + opiv_last_error());
+                log_error(load_error());
                 return 1;
             }
             px16[i].assign(tmp.begin(), tmp.end());
@@ -126,4 +133,17 @@ int main(int argc, char* argv[])
         os.write(reinterpret_cast<const char*>(be.data()), (std::streamsize)be.size());
     }
     return 0;
+}
+
+} // namespace
+
+int main(int argc, char* argv[])
+{
+    // (a compressed frame may declare more pixels than memory holds)
+    try {
+        return run(argc, argv);
+    } catch (const std::exception& e) {
+        log_error(std::string("failed: ") + e.what());
+        return 1;
+    }
 }

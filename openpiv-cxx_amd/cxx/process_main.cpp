@@ -167,8 +167,14 @@ int main(int argc, char* argv[])
 
     Frame fa, fb;
     std::string err;
-    if (!load_frame(inputs[0], fa, err) || !load_frame(inputs[1], fb, err)) {
-        log(Level::ERROR, err);
+    try {
+        // (a compressed frame may declare more pixels than memory holds)
+        if (!load_frame(inputs[0], fa, err) || !load_frame(inputs[1], fb, err)) {
+            log(Level::ERROR, err);
+            return 1;
+        }
+    } catch (const std::exception& e) {
+        log(Level::ERROR, std::string("failed to load the input images: ") + e.what());
         return 1;
     }
     if (fa.width != fb.width || fa.height != fb.height) {
@@ -218,7 +224,7 @@ int main(int argc, char* argv[])
 
         // main.cpp:312-313
         for (const auto& r : results) std::cout << r.x << ", " << r.y << ", " << r.u << ", " << r.v << ", " << r.sn << "\n";
-    } catch (const openpiv_amd::error& e) {
+    } catch (const std::exception& e) {   // openpiv_amd::error is a std::runtime_error
         log(Level::ERROR, e.what());
         return 1;
     }

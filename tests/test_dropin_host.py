@@ -58,3 +58,28 @@ def test_core_types_dropin_compiles_and_checks_sizes():
     subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "_ref/dropin_core"], check=True, timeout=600)
     r = run([DROPIN_CORE, "check"])
     assert r.returncode == 0 and r.stdout.strip() == "check ok", r.stdout + r.stderr
+
+
+def test_front_ends_fail_cleanly_on_huge_declared_frames(tmp_path):
+    """A 90-byte Deflate TIFF that declares a 2^31 x 2^31 16-bit frame: the
+    loader refuses the size, and both front-ends report it once ("failed to
+    load image: ...", as examples/average_subtract/main.cpp:123 words it)
+    and exit 1; a frame buffer that cannot be allocated would end the same
+    way (both catch std::exception)"""
+    import struct
+    import zlib
+    from dropin_util import AVGSUB_BIN
+    w = h = 2 ** 31
+    data = zlib.compress(bytes(64))
+    entries = [(256, 4, 1, w), (257, 4, 1, h), (258, 3, 1, 16), (259, 3, 1, 8), (262, 3, 1, 1),
+               (273, 4, 1, 8 + 2 + 9 * 12 + 4), (277, 3, 1, 1), (278, 4, 1, h), (279, 4, 1, len(data))]
+    ifd = struct.pack("<H", len(entries))
+    for t, ty, c, v in entries:
+        ifd += struct.pack("<HHIHH", t, ty, c, v, 0) if ty == 3 else struct.pack("<HHII", t, ty, c, v)
+    ifd += struct.pack("<I", 0)
+    path = tmp_path / "huge.tif"
+    path.write_bytes(b"II\x2a\x00" + struct.pack("<I", 8) + ifd + data)
+    for args in ([PROCESS_BIN, path, path], [AVGSUB_BIN, path, path]):
+        r = run(args)
+        assert r.returncode == 1, (args[0].name, r.returncode, r.stderr[-300:])
+        assert r.stderr.count("failed to load image: ") == 1 and "too large" in r.stderr, r.stderr
