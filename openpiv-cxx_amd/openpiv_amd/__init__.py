@@ -571,14 +571,16 @@ class DeviceBuffer:
 
     def upload(self, a: np.ndarray) -> None:
         a = np.ascontiguousarray(a)
-        assert a.nbytes <= self.nbytes
+        if a.nbytes > self.nbytes:
+            raise ValueError(f"{a.nbytes} bytes do not fit the {self.nbytes}-byte device buffer")
         if self._h.hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, self.H2D) != 0:
             raise OpivError(5, "hipMemcpy H2D failed")
         if self._h.hipDeviceSynchronize() != 0:
             raise OpivError(5, "hipDeviceSynchronize failed")
 
     def download(self, out: np.ndarray) -> np.ndarray:
-        assert out.flags.c_contiguous and out.nbytes <= self.nbytes
+        if not out.flags.c_contiguous or out.nbytes > self.nbytes:
+            raise ValueError(f"out must be a contiguous array of at most {self.nbytes} bytes")
         if self._h.hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, self.D2H) != 0:
             raise OpivError(5, "hipMemcpy D2H failed")
         return out

@@ -77,3 +77,12 @@ def test_python_layer_checks_host_sizes(gpu_ctx):
         plan.vjp(full, full, np.zeros(plan.count))
     res = plan.run(full, full)
     assert len(res) == plan.count
+    # device buffers refuse host arrays larger than themselves (not asserts:
+    # they hold under python -O)
+    buf = oa.DeviceBuffer(64)
+    with pytest.raises(ValueError):
+        buf.upload(np.zeros(17, np.float32))
+    with pytest.raises(ValueError):
+        buf.download(np.zeros(17, np.float32))
+    with pytest.raises(ValueError):
+        buf.download(np.zeros((4, 8), np.float32)[:, ::2])
